@@ -1,0 +1,110 @@
+/*
+ * C ABI of the gfx950 (MI355X) GETT backend of TCCG.
+ *
+ * Two kinds of entry points exist:
+ *
+ * 1. Per-contraction entry points, emitted by the generator for one .tccg
+ *    contraction with every extent fixed at generation time.  They keep the
+ *    reference's generated signature unchanged
+ *        void gett(T* A, T* B, T* C, const T alpha, const T beta)
+ *    (reference: tccg/gett.py:342-344, emitted at tccg/gett.py:1240-1243 and
+ *    declared in gett.hpp by tccg/gett.py:1405-1421; called positionally as
+ *    variant(A, B, C, alpha, beta) by the harness, tccg/tccg.py:1040).  A, B
+ *    and C are DEVICE pointers to dense column-major tensors (leftmost index
+ *    stride-one, tccg/tensor.py:8-13); the call is asynchronous on the null
+ *    stream.  The generated library additionally exports gett_stream(),
+ *    which reports errors and takes a stream, and gett_info().
+ *    These are declared by the generated header, see tccg_gett_generated.h.
+ *
+ * 2. The prebuilt, plan-driven library (libtccg_gfx950.so, declared here).
+ *    It runs any contraction whose offsets the host planner has tabulated
+ *    (tccg_gett_plan).  It replaces the same generated gett() for callers
+ *    that cannot compile per contraction (batched problems, adjoint
+ *    contractions generated at run time, shapes outside the reference's
+ *    multiple-of-24 limit, README.md:121-124 of the reference).
+ *
+ * All functions return 0 on success and a nonzero code otherwise; the text
+ * of the last error of the calling thread is available from
+ * tccg_last_error().  Bad arguments are rejected before any launch.
+ */
+#ifndef TCCG_GFX950_H
+#define TCCG_GFX950_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of a GETT launch. */
+enum tccg_prec {
+  TCCG_F32 = 0,    /* FP32 MFMA, FP32 accumulate (--floatType=s)            */
+  TCCG_F64 = 1,    /* FP64 MFMA (--floatType=d)                             */
+  TCCG_BF16X3 = 2, /* FP32 operands as three BF16 parts, 6 BF16 MFMA products */
+  TCCG_FP8 = 3     /* FP32 operands scaled per tensor to OCP e4m3, FP8 MFMA  */
+};
+
+/* How an operand tile is gathered (see tccg_amd/csrc/gett_core.hpp). */
+enum tccg_load_mode { TCCG_FREE4 = 0, TCCG_FREE1 = 1, TCCG_KDIM4 = 2, TCCG_KDIM1 = 3 };
+
+#define TCCG_INVALID 0xffffffffu
+
+/*
+ * Offset tables and launch shape of one contraction
+ *     C[m,n] = alpha * sum_k A[m,k] * B[k,n] + beta * C[m,n]
+ * after the reference's m/n/k classification (tccg/tccg_util.py:212-296,
+ * tccg/tccg_util.py:597-611) and its A/B swap, which makes A the operand that
+ * holds C's stride-one index (tccg/gett.py:70-76).  Element offsets are
+ * separable: off_A(m,k) = offAm[m] + offAk[k], off_B(k,n) = offBk[k] + offBn[n],
+ * off_C(m,n) = offCm[m] + offCn[n].  Tables are device arrays of uint32 whose
+ * lengths Mp, Np, Kp are whole multiples of the tile shape (tccg_tile_shape());
+ * the entry TCCG_INVALID marks a position holding no element (padding of a
+ * blocked linearisation): loads read zero there and C is not written.
+ * ordA/ordB (device int32, lengths BM/BN) give the tile row filled by each
+ * loader slot, so that the lanes of a wave can walk an operand's stride-one
+ * index; use the identity when no reordering is wanted.
+ */
+typedef struct tccg_gett_plan {
+  const uint32_t* offAm;
+  const uint32_t* offAk;
+  const uint32_t* offBn;
+  const uint32_t* offBk;
+  const uint32_t* offCm;
+  const uint32_t* offCn;
+  const int32_t* ordA;
+  const int32_t* ordB;
+  int32_t Mp, Np, Kp;  /* table lengths                                      */
+  int32_t tile;        /* tile configuration id, see tccg_tile_shape()       */
+  int32_t modeA;       /* enum tccg_load_mode of A                           */
+  int32_t modeB;       /* enum tccg_load_mode of B                           */
+  int32_t groupM;      /* m-tiles per scheduling group (>= 1)                */
+  const float* amax;   /* TCCG_FP8 only: device float[2] = max|A|, max|B|   */
+} tccg_gett_plan;
+
+/* Tile shape (BM x BN x BK) of tile configuration `tile` for precision `prec`. */
+int tccg_tile_shape(int prec, int tile, int* BM, int* BN, int* BK);
+
+/* Number of tile configurations defined for `prec` (ids 0..n-1). */
+int tccg_num_tiles(int prec);
+
+/* Plan-driven GETT.  stream is a hipStream_t (NULL = null stream). */
+int tccg_gett_f32(const float* A, const float* B, float* C, float alpha, float beta,
+                  const tccg_gett_plan* plan, void* stream);
+int tccg_gett_f64(const double* A, const double* B, double* C, double alpha, double beta,
+                  const tccg_gett_plan* plan, void* stream);
+int tccg_gett_bf16x3(const float* A, const float* B, float* C, float alpha, float beta,
+                     const tccg_gett_plan* plan, void* stream);
+int tccg_gett_fp8(const float* A, const float* B, float* C, float alpha, float beta,
+                  const tccg_gett_plan* plan, void* stream);
+
+/* out[0] = max_i |x[i]| (device float); used for the FP8 per-tensor scales. */
+int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tccg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCCG_GFX950_H */

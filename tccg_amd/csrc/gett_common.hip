@@ -1,0 +1,74 @@
+// Shared parts of the plan-driven GETT library: error reporting, tile shape
+// queries and the max|x| reduction that feeds the FP8 per-tensor scales.
+#include "gett_launch.hpp"
+
+namespace tccg_lib {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code ? code : 1;
+}
+
+namespace {
+__global__ void amax_zero_kernel(unsigned* out) {
+  if (threadIdx.x == 0) out[0] = 0u;
+}
+
+// |x| is non-negative, so its IEEE bit pattern orders like an unsigned integer.
+__global__ void amax_kernel(const float* __restrict__ x, int64_t n, unsigned* out) {
+  __shared__ float red[4];
+  float m = 0.f;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = fmaxf(m, fabsf(x[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = fmaxf(b, red[i]);
+    atomicMax(out, __float_as_uint(b));
+  }
+}
+
+}  // namespace
+
+}  // namespace tccg_lib
+
+extern "C" {
+
+int tccg_num_tiles(int prec) {
+  int n = 0;
+  tccg_lib::tiles_of(prec, &n);
+  return n;
+}
+
+int tccg_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
+  int n = 0;
+  const tccg_lib::TileCfg* t = tccg_lib::tiles_of(prec, &n);
+  if (!t) return tccg_lib::fail(1, "unknown precision");
+  if (tile < 0 || tile >= n) return tccg_lib::fail(1, "invalid tile id");
+  if (BM) *BM = t[tile].BM;
+  if (BN) *BN = t[tile].BN;
+  if (BK) *BK = t[tile].BK;
+  return 0;
+}
+
+int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream) {
+  using namespace tccg_lib;
+  if (!x || !out || n <= 0) return fail(1, "amax: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(amax_zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out));
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(amax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, reinterpret_cast<unsigned*>(out));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("amax launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+const char* tccg_last_error(void) { return tccg_lib::g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+"""GETT planning for the gfx950 target.
+
+A contraction C = alpha * A.B + beta * C is classified exactly as the
+reference's GETT generator does it (tccg/gett.py:60-89): the free indices of
+the input that holds C's stride-one index form the m set (C order), the free
+indices of the other input the n set, the contracted indices the k set (A
+order), and that input becomes the internal operand "A" (the A/B swap of
+tccg/gett.py:70-76).  Batch indices (present in A, B and C) are rejected as
+in the reference (tccg/gett.py:42-45).
+
+Where the reference splits each index set into cache-sized sub-sets
+(splitIndexSet, tccg/tccg_util.py:463-552) and packs the sub-tensors with
+HPTT, the gfx950 plan linearises each set (optionally blocked into
+rectangular sub-tensor tiles), tabulates every tensor's offsets along that
+linearisation, and hands the tables to the kernel, whose tile loader gathers
+along the operand's stride-one index (see csrc/gett_core.hpp).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .algebra import get_contracted_indices, get_loop_indices, get_m_indices, get_n_indices, product
+from .ir import Index, Tensor
+
+INVALID = np.uint32(0xFFFFFFFF)
+
+# Precisions (csrc/gett_core.hpp, include/tccg_gfx950.h)
+F32, F64, BF16X3, FP8 = 0, 1, 2, 3
+PREC_NAMES = {F32: "f32", F64: "f64", BF16X3: "bf16x3", FP8: "fp8"}
+PREC_BY_NAME = {v: k for k, v in PREC_NAMES.items()}
+# Load modes
+FREE4, FREE1, KDIM4, KDIM1 = 0, 1, 2, 3
+
+# Tile shapes per precision: (BM, BN, BK, LDS bytes per workgroup).  Must match
+# the tables of csrc/gett_launch.hpp (checked against the library at load time).
+TILES: Dict[int, List[Tuple[int, int, int]]] = {
+    F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16)],
+    F64: [(128, 128, 16), (64, 64, 16), (192, 128, 16)],
+    BF16X3: [(128, 128, 32), (64, 64, 32), (192, 128, 32)],
+    FP8: [(128, 128, 128), (64, 64, 128), (192, 128, 128)],
+}
+_LDS_PAD = {F32: (4, 4), F64: (2, 8), BF16X3: (8, 2), FP8: (16, 1)}  # (pad elements, element bytes)
+_PLANES = {F32: 1, F64: 1, BF16X3: 3, FP8: 1}
+
+
+def lds_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
+    pad, eb = _LDS_PAD[prec]
+    return 2 * _PLANES[prec] * (BM + BN) * (BK + pad) * eb
+
+
+class UnsupportedContraction(ValueError):
+    """The contraction cannot be expressed as a GETT (e.g. batch indices)."""
+
+
+# ---------------------------------------------------------------------------
+# Problem classification
+# ---------------------------------------------------------------------------
+class GettProblem:
+    """Classified contraction: internal operands opA (holds C's stride-one index)
+    and opB, index sets m, n, k and their extent products M, N, K."""
+
+    def __init__(self, C: Tensor, A: Tensor, B: Tensor):
+        loop = get_loop_indices(A, B, C)
+        if loop:
+            raise UnsupportedContraction("loop indices are not yet supported by GETT: %s"
+                                         % ",".join(i.label for i in loop))
+        for i in C.indices:
+            if not (A.has_index(i) or B.has_index(i)):
+                raise UnsupportedContraction("output index %s appears in no input" % i.label)
+        for X, Y in ((A, B), (B, A)):
+            for i in X.indices:
+                if not (C.has_index(i) or Y.has_index(i)):
+                    raise UnsupportedContraction("index %s of %s appears in neither the output nor the "
+                                                 "other input" % (i.label, X.label))
+        for T in (A, B, C):
+            labels = T.labels()
+            if len(set(labels)) != len(labels):
+                raise UnsupportedContraction("repeated index in %s" % T)
+        for X in (A, B):
+            for i in X.indices:
+                for T in (A, B, C):
+                    j = T.pos(i)
+                    if j >= 0 and T.indices[j].size != i.size:
+                        raise UnsupportedContraction("index %s has inconsistent extents" % i.label)
+        self.C, self.A, self.B = C, A, B
+        self.m = get_m_indices(A, B, C)
+        self.n = get_n_indices(A, B, C)
+        self.k = get_contracted_indices(A, B, C)
+        self.swapped = bool(self.n) and A.has_index(self.n[0])
+        self.opA, self.opB = (B, A) if self.swapped else (A, B)
+        self.M, self.N, self.K = product(self.m), product(self.n), product(self.k)
+
+    def flops(self, beta: float = 0.0) -> float:
+        """Algorithmic flop count of the reference (tccg/tccg.py:1230-1234)."""
+        f = 2.0 * self.M * self.N * self.K
+        if beta != 0:
+            f += 3.0 * self.M * self.N
+        return f
+
+    def __str__(self) -> str:
+        return "%s = %s * %s  (m=%s n=%s k=%s, M=%d N=%d K=%d)" % (
+            self.C, self.A, self.B, [i.label for i in self.m], [i.label for i in self.n],
+            [i.label for i in self.k], self.M, self.N, self.K)
+
+
+# ---------------------------------------------------------------------------
+# Linearisations and offset tables
+# ---------------------------------------------------------------------------
+@dataclass
+class Linearization:
+    """Order of the positions of one index set.
+
+    `labels`/`extents` list the set's indices; `inner` lists (label, block)
+    levels forming one rectangular sub-tensor tile of prod(blocks) positions
+    (first level fastest).  Indices absent from `inner` have block 1.  Tiles
+    follow each other in mixed-radix order over `labels`.  Positions whose
+    coordinate exceeds an extent are invalid.  With inner = all labels at
+    full extent this is the plain column-major order of the set.
+    """
+    labels: List[str]
+    extents: List[int]
+    inner: List[Tuple[str, int]]
+
+    @staticmethod
+    def plain(labels: Sequence[str], extents: Sequence[int]) -> "Linearization":
+        return Linearization(list(labels), list(extents), [(l, e) for l, e in zip(labels, extents)])
+
+    @property
+    def block(self) -> int:
+        return int(np.prod([b for _, b in self.inner])) if self.inner else 1
+
+    def is_plain(self) -> bool:
+        return self.inner == [(l, e) for l, e in zip(self.labels, self.extents)]
+
+    def coords(self, pad_to: int = 1) -> Tuple[np.ndarray, np.ndarray]:
+        """(coords[P, L] int64, valid[P] bool) with P padded to a multiple of pad_to."""
+        L = len(self.labels)
+        bsize = {l: 1 for l in self.labels}
+        for l, b in self.inner:
+            bsize[l] = b
+        T = self.block
+        nblk = [-(-e // bsize[l]) for l, e in zip(self.labels, self.extents)]
+        P0 = T * int(np.prod(nblk)) if L else 1
+        P = -(-P0 // pad_to) * pad_to
+        p = np.arange(P, dtype=np.int64)
+        coord = np.zeros((P, L), dtype=np.int64)
+        col = {l: i for i, l in enumerate(self.labels)}
+        inner = p % T
+        outer = p // T
+        for l, b in self.inner:
+            coord[:, col[l]] += inner % b
+            inner //= b
+        for i, l in enumerate(self.labels):
+            coord[:, i] += (outer % nblk[i]) * bsize[l]
+            outer //= nblk[i]
+        valid = p < P0
+        for i, e in enumerate(self.extents):
+            valid &= coord[:, i] < e
+        return coord, valid
+
+    def offsets(self, tensor: Tensor, pad_to: int = 1) -> np.ndarray:
+        """uint32 element offsets of `tensor` along this linearisation; INVALID
+        where a position holds no element."""
+        coord, valid = self.coords(pad_to)
+        off = np.zeros(coord.shape[0], dtype=np.int64)
+        for i, l in enumerate(self.labels):
+            s = tensor.stride_of_label(l)
+            if s >= 0:
+                off += coord[:, i] * s
+        if off.size and off.max(initial=0) >= 0xFFFFFFFF:
+            raise UnsupportedContraction("tensor %s exceeds 2^32-1 elements" % tensor.label)
+        out = off.astype(np.uint32)
+        out[~valid] = INVALID
+        return out
+
+
+def _divisors(n: int) -> List[int]:
+    return [d for d in range(1, n + 1) if n % d == 0]
+
+
+def blocked_linearization(labels: Sequence[str], extents: Sequence[int], first: Sequence[str],
+                          tile: int) -> Optional[Linearization]:
+    """Rectangular tiles of exactly `tile` positions, giving block sizes to the
+    labels of `first` (in order) and then to the remaining labels, each block a
+    divisor of what is left of `tile` chosen to waste the fewest positions."""
+    ext = dict(zip(labels, extents))
+    order = [l for l in first if l in ext] + [l for l in labels if l not in first]
+    remaining = tile
+    inner: List[Tuple[str, int]] = []
+    for l in order:
+        if remaining == 1:
+            break
+        e = ext[l]
+        best = None
+        for d in _divisors(remaining):
+            if d > e and d != 1:
+                continue
+            waste = (-(-e // d) * d) / e
+            key = (round(waste, 9), -d)
+            if best is None or key < best[0]:
+                best = (key, d)
+        d = best[1]
+        if d > 1:
+            inner.append((l, d))
+            remaining //= d
+    if remaining != 1:
+        return None
+    return Linearization(list(labels), list(extents), inner)
+
+
+def _vector_ok_free(off: np.ndarray, ordr: np.ndarray, F: int) -> bool:
+    """Every aligned group of 4 loader slots of every tile is either all invalid
+    or 4 consecutive offsets starting at a multiple of 4."""
+    if F % 4:
+        return False
+    t = off.reshape(-1, F)[:, ordr].reshape(-1, 4).astype(np.int64)
+    inv = t == np.int64(INVALID)
+    all_inv = inv.all(axis=1)
+    any_inv = inv.any(axis=1)
+    if np.any(any_inv & ~all_inv):
+        return False
+    t = t[~all_inv]
+    if t.size == 0:
+        return True
+    return bool(np.all(t[:, 0] % 4 == 0) and np.all(t - t[:, :1] == np.arange(4)))
+
+
+def _vector_ok_k(off: np.ndarray) -> bool:
+    if off.size % 4:
+        return False
+    return _vector_ok_free(off, np.arange(off.size), off.size)
+
+
+def _loader_order(lin: Linearization, off: np.ndarray, F: int) -> np.ndarray:
+    """Loader slot -> tile row.  For blocked tiles the slots follow the operand's
+    memory order inside a tile (every full tile has the same relative layout);
+    plain orders keep the identity."""
+    if lin.is_plain() or lin.block != F:
+        return np.arange(F, dtype=np.int32)
+    first = off[:F].astype(np.int64)
+    key = np.where(first == np.int64(INVALID), np.iinfo(np.int64).max, first)
+    return np.argsort(key, kind="stable").astype(np.int32)
+
+
+@dataclass
+class GettPlan:
+    """Everything the plan-driven kernel needs for one contraction."""
+    problem: GettProblem
+    prec: int
+    tile: int
+    BM: int
+    BN: int
+    BK: int
+    lin_m: Linearization
+    lin_n: Linearization
+    lin_k: Linearization
+    offAm: np.ndarray
+    offAk: np.ndarray
+    offBn: np.ndarray
+    offBk: np.ndarray
+    offCm: np.ndarray
+    offCn: np.ndarray
+    ordA: np.ndarray
+    ordB: np.ndarray
+    modeA: int
+    modeB: int
+    groupM: int = 8
+    notes: List[str] = field(default_factory=list)
+
+    @property
+    def Mp(self) -> int:
+        return self.offAm.size
+
+    @property
+    def Np(self) -> int:
+        return self.offBn.size
+
+    @property
+    def Kp(self) -> int:
+        return self.offAk.size
+
+    def ntiles(self) -> int:
+        return (self.Mp // self.BM) * (self.Np // self.BN)
+
+    def padded_flops(self) -> float:
+        return 2.0 * self.Mp * self.Np * self.Kp
+
+    def describe(self) -> str:
+        mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
+        return ("prec=%s tile=%d (%dx%dx%d) m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
+                % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, self.lin_m.inner,
+                   self.lin_n.inner, self.lin_k.inner, mode[self.modeA], mode[self.modeB], self.ntiles(),
+                   self.Mp, self.Np, self.Kp))
+
+
+def _orders(pb: GettProblem) -> Tuple[List[str], List[str], List[str]]:
+    """Index orders of the m, n and k sets.
+
+    m keeps C's order, so C's stride-one index is fastest and C is written
+    along its stride-one index.  n puts opB's stride-one index first when it is
+    free.  k puts opA's stride-one index first when it is contracted, else
+    opB's, else keeps A's order.
+    """
+    m = [i.label for i in pb.m]
+    n = [i.label for i in pb.n]
+    b0 = pb.opB.indices[0].label
+    if b0 in n:
+        n = [b0] + [l for l in n if l != b0]
+    k = [i.label for i in pb.k]
+    a0 = pb.opA.indices[0].label
+    if a0 in k:
+        k = [a0] + [l for l in k if l != a0]
+    elif b0 in k:
+        k = [b0] + [l for l in k if l != b0]
+    return m, n, k
+
+
+def _ext(pb: GettProblem, labels: Sequence[str]) -> List[int]:
+    out = []
+    for l in labels:
+        for T in (pb.C, pb.opA, pb.opB):
+            p = T.pos(Index(l, 0))
+            if p >= 0:
+                out.append(T.indices[p].size)
+                break
+    return out
+
+
+def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] = None,
+               groupM: int = 8) -> GettPlan:
+    BM, BN, BK = TILES[prec][tile]
+    m_l, n_l, k_l = _orders(pb)
+    m_e, n_e, k_e = _ext(pb, m_l), _ext(pb, n_l), _ext(pb, k_l)
+    c0 = pb.C.indices[0].label
+    a0 = pb.opA.indices[0].label
+    conflict = a0 in m_l and a0 != c0
+    if m_blocked is None:
+        m_blocked = conflict
+    lin_m = None
+    if m_blocked and m_l:
+        lin_m = blocked_linearization(m_l, m_e, [c0, a0] if conflict else [c0], BM)
+    if lin_m is None:
+        lin_m = Linearization.plain(m_l, m_e)
+    lin_n = Linearization.plain(n_l, n_e)
+    lin_k = Linearization.plain(k_l, k_e)
+
+    offAm = lin_m.offsets(pb.opA, BM)
+    offCm = lin_m.offsets(pb.C, BM)
+    offBn = lin_n.offsets(pb.opB, BN)
+    offCn = lin_n.offsets(pb.C, BN)
+    offAk = lin_k.offsets(pb.opA, BK)
+    offBk = lin_k.offsets(pb.opB, BK)
+    ordA = _loader_order(lin_m, offAm, BM)
+    ordB = _loader_order(lin_n, offBn, BN)
+
+    def mode(X: Tensor, free_labels: List[str], off_free, ordr, F, off_k) -> int:
+        s = X.indices[0].label
+        if s in free_labels:
+            return FREE4 if _vector_ok_free(off_free, ordr, F) else FREE1
+        if s in k_l:
+            return KDIM4 if _vector_ok_k(off_k) else KDIM1
+        return FREE1
+
+    modeA = mode(pb.opA, m_l, offAm, ordA, BM, offAk)
+    modeB = mode(pb.opB, n_l, offBn, ordB, BN, offBk)
+    return GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
+                    ordA, ordB, modeA, modeB, groupM)
+
+
+# ---------------------------------------------------------------------------
+# Performance model (gfx950 analogue of Gett.estimateGFLOPS, tccg/gett.py:590-687)
+# ---------------------------------------------------------------------------
+MI355X = {
+    "cus": 256,
+    "clock_ghz": 2.4,
+    "hbm_bytes_per_s": 8.0e12,
+    "lds_bytes_per_cu": 160 * 1024,
+    # dense matrix peaks, FLOP/s
+    "peak": {F32: 157.3e12, F64: 78.6e12, BF16X3: 2516.6e12 / 6.0, FP8: 5033.0e12},
+}
+
+
+def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
+    """Roofline estimate of one launch: max(MFMA time of the padded work with
+    wave quantisation over the CUs, HBM time of A, B and C), with a penalty for
+    small tiles (more L2 traffic per flop) and scalar (non-vector) loads."""
+    pb = plan.problem
+    hw = MI355X
+    lds = lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK)
+    per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
+    slots = hw["cus"] * per_cu
+    ntiles = plan.ntiles()
+    waves = math.ceil(ntiles / slots)
+    per_tile = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots)
+    eff = 0.75 if plan.BM * plan.BN >= 128 * 128 else 0.55
+    if plan.modeA in (FREE1, KDIM1):
+        eff *= 0.9
+    if plan.modeB in (FREE1, KDIM1):
+        eff *= 0.9
+    t_mfma = waves * per_tile / eff
+    esz = 8 if plan.prec == F64 else 4
+    bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
+    t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
+    return max(t_mfma, t_hbm)
+
+
+def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPlan]:
+    """Every (tile, blocking) plan for the problem, best model estimate first."""
+    plans = []
+    c0 = pb.C.indices[0].label
+    a0 = pb.opA.indices[0].label
+    conflict = a0 in [i.label for i in pb.m] and a0 != c0
+    for tile in range(len(TILES[prec])):
+        for blocked in ((False, True) if conflict else (False,)):
+            plans.append(build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM))
+    plans.sort(key=lambda p: estimate_seconds(p))
+    return plans
+
+
+def default_plan(pb: GettProblem, prec: int) -> GettPlan:
+    return candidate_plans(pb, prec)[0]

@@ -1,0 +1,160 @@
+"""ctypes layer between Python/PyTorch-ROCm and the gfx950 GETT library.
+
+The library (tccg_amd/_lib/libtccg_gfx950.so, built by tccg_amd.build) exports
+the C ABI of include/tccg_gfx950.h.  This module uploads a plan's offset
+tables to the device once, and launches GETT on torch tensors (device
+pointers from Tensor.data_ptr()) on torch's current HIP stream.  There is no
+CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import build as _build
+from .plan import F32, F64, BF16X3, FP8, GettPlan, TILES
+
+
+class GettRuntimeError(RuntimeError):
+    pass
+
+
+class _CPlan(ctypes.Structure):
+    _fields_ = [
+        ("offAm", ctypes.c_void_p), ("offAk", ctypes.c_void_p),
+        ("offBn", ctypes.c_void_p), ("offBk", ctypes.c_void_p),
+        ("offCm", ctypes.c_void_p), ("offCn", ctypes.c_void_p),
+        ("ordA", ctypes.c_void_p), ("ordB", ctypes.c_void_p),
+        ("Mp", ctypes.c_int32), ("Np", ctypes.c_int32), ("Kp", ctypes.c_int32),
+        ("tile", ctypes.c_int32), ("modeA", ctypes.c_int32), ("modeB", ctypes.c_int32),
+        ("groupM", ctypes.c_int32),
+        ("amax", ctypes.c_void_p),
+    ]
+
+
+EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_gett_f64",
+                    "tccg_gett_bf16x3", "tccg_gett_fp8", "tccg_amax_f32", "tccg_last_error"]
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
+    """Load (once) the in-tree GETT library; builds it only if it is absent."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        p = Path(path) if path else _build.library_path()
+        if not p.exists():
+            if not _build.hipcc_available():
+                raise GettRuntimeError("GETT library %s is missing and hipcc is unavailable" % p)
+            _build.build_library()
+        lib = ctypes.CDLL(str(p))
+        vp, i32, f32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double
+        lib.tccg_tile_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        lib.tccg_num_tiles.argtypes = [ctypes.c_int]
+        for name, T in (("tccg_gett_f32", f32), ("tccg_gett_bf16x3", f32), ("tccg_gett_fp8", f32),
+                        ("tccg_gett_f64", f64)):
+            fn = getattr(lib, name)
+            fn.argtypes = [vp, vp, vp, T, T, ctypes.POINTER(_CPlan), vp]
+            fn.restype = ctypes.c_int
+        lib.tccg_amax_f32.argtypes = [vp, ctypes.c_int64, vp, vp]
+        lib.tccg_last_error.restype = ctypes.c_char_p
+        # The Python tile table must agree with the library's.
+        for prec, tiles in TILES.items():
+            n = lib.tccg_num_tiles(prec)
+            if n != len(tiles):
+                raise GettRuntimeError("tile table mismatch for precision %d: library %d, host %d"
+                                       % (prec, n, len(tiles)))
+            for t, shape in enumerate(tiles):
+                bm, bn, bk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+                lib.tccg_tile_shape(prec, t, ctypes.byref(bm), ctypes.byref(bn), ctypes.byref(bk))
+                if (bm.value, bn.value, bk.value) != shape:
+                    raise GettRuntimeError("tile %d/%d shape mismatch" % (prec, t))
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    return load_library().tccg_last_error().decode()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _check_cuda():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise GettRuntimeError("the gfx950 GETT path needs a ROCm GPU; none is visible")
+    return torch
+
+
+class DevicePlan:
+    """A GettPlan whose tables live on the GPU."""
+
+    def __init__(self, plan: GettPlan, device=None):
+        torch = _check_cuda()
+        self.plan = plan
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+
+        def up(a: np.ndarray, dt):
+            return torch.from_numpy(np.ascontiguousarray(a).view(dt)).to(dev)
+
+        i32 = np.int32
+        self.tables = [up(plan.offAm, i32), up(plan.offAk, i32), up(plan.offBn, i32), up(plan.offBk, i32),
+                       up(plan.offCm, i32), up(plan.offCn, i32), up(plan.ordA, i32), up(plan.ordB, i32)]
+        self.amax = torch.zeros(2, dtype=torch.float32, device=dev) if plan.prec == FP8 else None
+        c = _CPlan()
+        (c.offAm, c.offAk, c.offBn, c.offBk, c.offCm, c.offCn, c.ordA, c.ordB) = [t.data_ptr() for t in self.tables]
+        c.Mp, c.Np, c.Kp = plan.Mp, plan.Np, plan.Kp
+        c.tile, c.modeA, c.modeB, c.groupM = plan.tile, plan.modeA, plan.modeB, plan.groupM
+        c.amax = self.amax.data_ptr() if self.amax is not None else None
+        self.cplan = c
+
+    def _operands(self, A, B, C):
+        """Tensors in the .tccg's (A, B, C) order -> kernel operands (opA, opB)."""
+        pb = self.plan.problem
+        torch = _torch()
+        want = torch.float64 if self.plan.prec == F64 else torch.float32
+        for name, t, T in (("A", A, pb.A), ("B", B, pb.B), ("C", C, pb.C)):
+            if not isinstance(t, torch.Tensor):
+                raise TypeError("%s must be a torch.Tensor" % name)
+            if t.device.type != "cuda":
+                raise GettRuntimeError("%s must be a GPU tensor" % name)
+            if t.dtype != want:
+                raise TypeError("%s has dtype %s, expected %s" % (name, t.dtype, want))
+            if not t.is_contiguous():
+                raise GettRuntimeError("%s must be contiguous (TCCG layout, leftmost index stride-one)" % name)
+            if t.numel() != T.size():
+                raise GettRuntimeError("%s has %d elements, the contraction needs %d" % (name, t.numel(), T.size()))
+        return (B, A) if pb.swapped else (A, B)
+
+    def run(self, A, B, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        """C <- alpha * contract(A, B) + beta * C, asynchronously on `stream`
+        (default: torch's current stream)."""
+        torch = _torch()
+        lib = load_library()
+        opA, opB = self._operands(A, B, C)
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        sptr = ctypes.c_void_p(s.cuda_stream)
+        prec = self.plan.prec
+        if prec == FP8:
+            rc = lib.tccg_amax_f32(opA.data_ptr(), opA.numel(), self.amax.data_ptr(), sptr)
+            if rc == 0:
+                rc = lib.tccg_amax_f32(opB.data_ptr(), opB.numel(), self.amax.data_ptr() + 4, sptr)
+            if rc != 0:
+                raise GettRuntimeError(last_error())
+        fn = {F32: lib.tccg_gett_f32, F64: lib.tccg_gett_f64, BF16X3: lib.tccg_gett_bf16x3,
+              FP8: lib.tccg_gett_fp8}[prec]
+        rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
+        if rc != 0:
+            raise GettRuntimeError("GETT launch failed: %s" % last_error())

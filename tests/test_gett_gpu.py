@@ -1,0 +1,100 @@
+"""Parity of the gfx950 GETT kernels with the FP64 einsum oracle, through the
+C ABI (tccg_amd.runtime -> libtccg_gfx950.so)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from tccg_amd import Contraction
+from tccg_amd.plan import BF16X3, F32, F64, FP8, FREE1, FREE4, KDIM1, KDIM4, TILES, build_plan
+from tccg_amd.workloads import WORKLOADS, scaled
+
+from .gpu_helpers import check
+
+pytestmark = pytest.mark.gpu
+
+SMALL = {
+    "example": scaled("example", 0.5),
+    "cpu48": scaled("cpu48", 0.25),
+    "abcd-aebf-dfce": scaled("abcd-aebf-dfce", 0.25),
+    "abc-akb-kc": scaled("abc-akb-kc", 0.125),
+    "abcdef-dgfa-gceb": scaled("abcdef-dgfa-gceb", 0.34),
+    "abcd-aklb-lckd": scaled("abcd-aklb-lckd", 0.25),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3, FP8])
+def test_workloads_small_every_tile(name, prec):
+    c = Contraction.from_tccg(SMALL[name])
+    for tile in range(len(TILES[prec])):
+        for blocked in (False, True):
+            plan = build_plan(c.problem, prec, tile, m_blocked=blocked)
+            check(c, prec, seed=tile, plan=plan)
+
+
+# ragged extents, permutations that exercise every load mode, vector and scalar
+PATTERNS = [
+    (["a", "b"], ["a", "k"], ["k", "b"]),        # GEMM NN: A FREE, B KDIM
+    (["a", "b"], ["k", "a"], ["k", "b"]),        # A KDIM
+    (["a", "b"], ["a", "k"], ["b", "k"]),        # B FREE
+    (["a", "b"], ["k", "a"], ["b", "k"]),        # both KDIM/FREE swapped
+    (["a", "b", "c"], ["c", "k", "a"], ["k", "b"]),  # A stride-one is not C's: blocked
+    (["b", "a"], ["a", "k"], ["k", "b"]),        # operands swapped internally
+    (["a", "b", "c", "d"], ["d", "k", "a", "l"], ["l", "c", "k", "b"]),
+    (["a"], ["a", "k"], ["k"]),                  # matrix-vector (empty n set)
+    (["a", "b"], ["a"], ["b"]),                  # outer product (empty k set)
+]
+
+
+@pytest.mark.parametrize("pi", range(len(PATTERNS)))
+@pytest.mark.parametrize("ext", [(5, 7, 3), (8, 12, 16), (33, 17, 65)])
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
+def test_patterns_ragged(pi, ext, prec):
+    c_l, a_l, b_l = PATTERNS[pi]
+    labels = sorted(set(c_l + a_l + b_l))
+    sizes = {l: ext[i % len(ext)] for i, l in enumerate(labels)}
+    c = Contraction.from_labels(c_l, a_l, b_l, sizes)
+    check(c, prec, seed=pi)
+
+
+def test_load_modes_covered():
+    """The pattern list reaches every load mode for A and B."""
+    seen = set()
+    for c_l, a_l, b_l in PATTERNS:
+        for ext in [(8, 12, 16), (5, 7, 3)]:
+            labels = sorted(set(c_l + a_l + b_l))
+            sizes = {l: ext[i % len(ext)] for i, l in enumerate(labels)}
+            c = Contraction.from_labels(c_l, a_l, b_l, sizes)
+            p = c.plan(F32)
+            seen.add(("A", p.modeA))
+            seen.add(("B", p.modeB))
+    for op in "AB":
+        for m in (FREE4, FREE1, KDIM4, KDIM1):
+            assert (op, m) in seen, (op, m, seen)
+
+
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (2.4, 1.1), (-0.5, 1.0), (1.0, -2.0)])
+def test_alpha_beta(prec, alpha, beta):
+    c = Contraction.from_tccg(scaled("abcd-aebf-dfce", 0.2))
+    check(c, prec, seed=3, alpha=alpha, beta=beta)
+
+
+def test_beta_from_tccg():
+    txt = """C[i1,j,i2] = 2.4 * A[i1,k,i2] * B[j,k] + 1.1 * C[i1,j,i2]
+i1 = 48
+i2 = 40
+j = 36
+k = 8
+"""
+    c = Contraction.from_tccg(txt)
+    assert (c.alpha, c.beta) == (2.4, 1.1)
+    check(c, F32, seed=5)
+
+
+def test_fused_indices_run():
+    # C[a,b,c] = A[a,b,k] * B[k,c] fuses (a,b) -> one index
+    c = Contraction.from_labels(["a", "b", "c"], ["a", "b", "k"], ["k", "c"], dict(a=12, b=10, c=20, k=9))
+    assert len(c.C.indices) == 2
+    check(c, F32, seed=7)
