@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 GETT path on one MI355X.
+
+A step is one GETT launch over one contraction whose operands are already
+resident in HBM.  The headline workload is the compute-bound plain-FP32
+contraction C[a,b,c,d] = A[a,e,b,f] * B[d,f,c,e] (a=b=c=d=72, e=f=48,
+alpha=1, beta=0, inputs iid uniform[-1,1]); the same JSON line carries the
+bandwidth-bound contraction C[a,b,c] = A[a,k,b] * B[k,c] (384/8), the FP64
+variant, the split-BF16 and FP8 forms with their error against FP64, the
+roofline of the dominant kernel and the CPU baseline.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GFLOP/s + roofline %; split-BF16 vs plain FP32 GETT at equal error vs FP64"
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "bf16x3": 2516.6 / 6.0, "fp8": 5033.0}
+PEAK_HBM_GBS = 8000.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--max-implementations", type=int, default=8,
+                    help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
+    ap.add_argument("--only", default=None, help="run one form only: f32, bw, f64, bf16x3, fp8 (profiling)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def rel_err(x, ref) -> float:
+    import torch
+    d = (x.double() - ref).norm()
+    return float(d / ref.norm())
+
+
+def fp64_reference(c, A, B):
+    """alpha*A.B in float64 on the GPU with torch.einsum over contiguous
+    operands (an independent library path, not this backend)."""
+    import string
+    import torch
+    letters = {}
+    pool = iter(string.ascii_letters)
+    for lab in [i.label for i in c.C0.indices + c.A0.indices + c.B0.indices]:
+        letters.setdefault(lab, next(pool))
+    rev = lambda T: "".join(letters[i.label] for i in reversed(T.indices))
+    eq = "%s,%s->%s" % (rev(c.A0), rev(c.B0), rev(c.C0))
+    return torch.einsum(eq, A.double().contiguous(), B.double().contiguous()) * c.alpha
+
+
+def timed_steps(dp, A, B, C, steps: int, warmup: int):
+    """(wall seconds of `steps` launches bracketed by synchronize, average
+    per-launch HIP-event duration in seconds)."""
+    import torch
+    s = torch.cuda.current_stream()
+    for _ in range(warmup):
+        dp.run(A, B, C)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(s)
+        dp.run(A, B, C)
+        e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = sum(e0.elapsed_time(e1) for e0, e1 in evs) / steps * 1e-3
+    return wall, kern
+
+
+def cold_min(dp, A, B, C, reps: int = 5) -> float:
+    """Minimum launch time after writing a 512 MiB buffer (evicts L2 and the
+    256 MiB Infinity Cache), the reference's trash-then-time method
+    (tccg/tccg.py:725-737, tccg/tccg.py:1032-1058)."""
+    import torch
+    trash = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    s = torch.cuda.current_stream()
+    best = float("inf")
+    for r in range(reps):
+        trash.fill_(float(r))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        dp.run(A, B, C)
+        e1.record(s)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del trash
+    return best
+
+
+def run_form(name, wl, prec, args, log):
+    import torch
+    from tccg_amd import Contraction
+    from tccg_amd.autotune import autotune
+    from tccg_amd.runtime import DevicePlan
+    from tccg_amd.workloads import get
+
+    c = Contraction.from_tccg(get(wl))
+    dt = torch.float64 if prec == "f64" else torch.float32
+    sa, sb, sc = c.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C = torch.zeros(sc, dtype=dt, device="cuda")
+    from tccg_amd.plan import PREC_BY_NAME
+    p = PREC_BY_NAME[prec]
+    t0 = time.perf_counter()
+    ms = autotune(c, p, max_implementations=args.max_implementations, reps=3, inputs=(A, B, C),
+                  verbose=args.verbose)
+    tune_s = time.perf_counter() - t0
+    best = ms[0].plan
+    dp = DevicePlan(best)
+    wall, kern = timed_steps(dp, A, B, C, args.steps, args.warmup)
+    cold = cold_min(dp, A, B, C)
+    flops = c.flops()
+    bytes_ = A.element_size() * (A.numel() + B.numel() + C.numel())
+    res = {
+        "workload": str(c), "prec": prec, "M": c.problem.M, "N": c.problem.N, "K": c.problem.K,
+        "plan": best.describe(), "tuned_candidates": len(ms), "tune_seconds": round(tune_s, 2),
+        "wall_ms_per_step": wall / args.steps * 1e3, "kernel_ms": kern * 1e3, "cold_min_ms": cold * 1e3,
+        "gflops": flops / (wall / args.steps) / 1e9, "kernel_gflops": flops / kern / 1e9,
+        "kernel_gbs": bytes_ / kern / 1e9, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+    }
+    if prec != "f64" and wl in ("abcd-aebf-dfce", "abcdef-dgfa-gceb"):
+        ref = fp64_reference(c, A, B)
+        res["rel_err_vs_fp64"] = rel_err(C, ref)
+        del ref
+    elif prec == "f64":
+        ref = fp64_reference(c, A, B)
+        res["rel_err_vs_fp64_einsum"] = rel_err(C, ref)
+        del ref
+    log("%-7s %-18s %9.1f GFLOP/s (kernel %.3f ms, cold %.3f ms) %s" % (
+        prec, wl, res["kernel_gflops"], res["kernel_ms"], res["cold_min_ms"], best.describe()))
+    del A, B, C
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(args, log):
+    """The oracle's AVX2 GETT restatement on the host cores, on a bounded sample
+    of the headline workload (whole contraction repeated for about
+    --cpu-sample-seconds)."""
+    try:
+        from oracle import avx2_gett
+    except Exception as e:  # the restatement is optional for the GPU line
+        log("cpu baseline unavailable: %s" % e)
+        return None
+    return avx2_gett.bench_workload("abcd-aebf-dfce", seconds=args.cpu_sample_seconds, log=log)
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the GETT backend runs one contraction per GPU and this build "
+              "has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    log = (lambda s: print(s, file=sys.stderr, flush=True))
+    forms = {
+        "f32": ("abcd-aebf-dfce", "f32"),
+        "bw": ("abc-akb-kc", "f32"),
+        "f64": ("abcd-aebf-dfce", "f64"),
+        "bf16x3": ("abcd-aebf-dfce", "bf16x3"),
+        "fp8": ("abcdef-dgfa-gceb", "fp8"),
+    }
+    if args.only:
+        forms = {args.only: forms[args.only]}
+    results = {}
+    for key, (wl, prec) in forms.items():
+        results[key] = run_form(key, wl, prec, args, log)
+    head = results.get("f32") or next(iter(results.values()))
+    details = {}
+    if "bf16x3" in results and "f32" in results:
+        details["split_bf16_speedup_vs_f32"] = results["bf16x3"]["kernel_gflops"] / results["f32"]["kernel_gflops"]
+        details["split_bf16_err_ratio_vs_f32"] = (results["bf16x3"]["rel_err_vs_fp64"]
+                                                   / max(results["f32"]["rel_err_vs_fp64"], 1e-30))
+    if "bw" in results:
+        details["bw_hbm_roofline_frac"] = results["bw"]["kernel_gbs"] / PEAK_HBM_GBS
+    cpu = None if (args.no_cpu_baseline or args.only) else cpu_baseline(args, log)
+    achieved_tf = head["kernel_gflops"] / 1e3
+    line = {
+        "metric": METRIC,
+        "value": round(head["gflops"], 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["wall_ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["prec"],
+        "data": "synthetic: iid uniform[-1,1] (torch generator seed 0), operands resident in HBM",
+        "config": {"workload": head["workload"], "M": head["M"], "N": head["N"], "K": head["K"],
+                   "alpha": 1.0, "beta": 0.0, "plan": head["plan"]},
+        "roofline": {"bound": "mfma" if head is not results.get("bw") else "hbm",
+                     "achieved": round(achieved_tf, 2), "peak": PEAK_TFLOPS[head["prec"]], "unit": "TFLOP/s",
+                     "frac": round(achieved_tf / PEAK_TFLOPS[head["prec"]], 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "forms": results,
+        "details": details,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

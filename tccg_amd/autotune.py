@@ -1,0 +1,75 @@
+"""GPU autotuning of GETT plans.
+
+Mirrors the reference's search (tccg/gett.py:987-1150 ranks candidates by the
+performance model and keeps the best maxImplementations; tccg/tccg.py:278-430
+times each one and keeps the fastest): candidate plans are ranked by
+plan.estimate_seconds, the first `max_implementations` are timed on the GPU
+with HIP events (minimum over repetitions), and the fastest is returned.
+Results can be persisted in the tuning database (tccg_amd.tuning_db).
+"""
+from __future__ import annotations
+
+import sys
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+from .plan import F64, GettPlan, candidate_plans, estimate_seconds
+
+
+@dataclass
+class Measurement:
+    plan: GettPlan
+    seconds: float
+    estimated: float
+
+    def gflops(self) -> float:
+        return self.plan.problem.flops() / self.seconds / 1e9
+
+
+def time_plan(dp, A, B, C, alpha=1.0, beta=0.0, reps: int = 5, warmup: int = 1) -> float:
+    """Minimum launch time (s) of a DevicePlan over `reps` launches, each timed
+    with HIP events on the launching stream."""
+    import torch
+    s = torch.cuda.current_stream()
+    for _ in range(warmup):
+        dp.run(A, B, C, alpha, beta)
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        dp.run(A, B, C, alpha, beta)
+        e1.record(s)
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    return best
+
+
+def autotune(contraction, prec: int, max_implementations: int = 16, reps: int = 5, inputs=None,
+             candidates: Optional[Sequence[GettPlan]] = None, verbose: bool = False) -> List[Measurement]:
+    """Time up to max_implementations model-ranked candidates; fastest first."""
+    import torch
+    from .runtime import DevicePlan
+    plans = list(candidates) if candidates is not None else candidate_plans(contraction.problem, prec)
+    if max_implementations >= 0:
+        plans = plans[:max_implementations]
+    if inputs is None:
+        dt = torch.float64 if prec == F64 else torch.float32
+        sa, sb, sc = contraction.shapes()
+        g = torch.Generator(device="cuda").manual_seed(0)
+        A = torch.rand(sa, dtype=dt, device="cuda", generator=g) * 2 - 1
+        B = torch.rand(sb, dtype=dt, device="cuda", generator=g) * 2 - 1
+        C = torch.zeros(sc, dtype=dt, device="cuda")
+    else:
+        A, B, C = inputs
+    out = []
+    for p in plans:
+        dp = DevicePlan(p)
+        t = time_plan(dp, A, B, C, reps=reps)
+        m = Measurement(p, t, estimate_seconds(p))
+        if verbose:
+            print("  %-100s %9.1f us  %8.1f GFLOP/s (estimated %.1f us)"
+                  % (p.describe()[:100], t * 1e6, m.gflops(), m.estimated * 1e6), file=sys.stderr, flush=True)
+        out.append(m)
+    out.sort(key=lambda m: m.seconds)
+    return out

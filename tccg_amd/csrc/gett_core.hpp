@@ -172,13 +172,18 @@ struct TileLoader {
   static constexpr bool FREE_LANES = (MODE == FREE4 || MODE == FREE1);
   static constexpr int NBLK = (F / VF) * (BK / 4);
   static constexpr int PT = (NBLK + 255) / 256;
+  static constexpr int NJ = (MODE == KDIM4) ? 1 : 4;  // k-table entries per micro-block
   static constexpr int LDK = BK + TR::PAD;
   static_assert(F % VF == 0 && BK % 4 == 0, "tile shape");
 
   TG v[PT][VF][4];
   uint32_t foff[PT];
+  uint32_t kn[PT][NJ];  // k offsets of the next k-step to load
+  unsigned okm[PT];     // bit j: k value j of the loaded micro-block is valid
   int row[PT][VF];
   int kl[PT];
+
+  __device__ static constexpr bool active(int t, int p) { return (NBLK % 256 == 0) || (t + 256 * p) < NBLK; }
 
   // Loader slots are walked lane-major; ord maps a slot to the tile row it
   // fills, so the lanes of a wave can follow the operand's stride-one index.
@@ -190,61 +195,69 @@ struct TileLoader {
       if (FREE_LANES) { fg = b % (F / VF); kg = b / (F / VF); }
       else { kg = b % (BK / 4); fg = b / (BK / 4); }
       kl[p] = kg * 4;
-      const bool ok = (NBLK % 256 == 0) || b < NBLK;
+      const bool ok = active(t, p);
 #pragma unroll
       for (int e = 0; e < VF; ++e) row[p][e] = ok ? ord[fg * VF + e] : 0;
       foff[p] = ok ? offF[f0 + row[p][0]] : kInvalid;
     }
   }
 
-  __device__ inline void load(const TG* __restrict__ X, const uint32_t* __restrict__ offK, int k0, int t) {
+  // Table entries of the k-step starting at k0 (issued one step ahead so that
+  // the data loads never wait on a table load).
+  __device__ inline void fetch_k(const uint32_t* __restrict__ offK, int k0, int t) {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
-      const int b = t + 256 * p;
-      if (NBLK % 256 != 0 && b >= NBLK) continue;
-      const int kb = k0 + kl[p];
+      if (!active(t, p)) continue;
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) kn[p][j] = offK[k0 + kl[p] + j];
+    }
+  }
+
+  // Branch-free gathers: an invalid position reads element 0 and is zeroed
+  // when it is written to LDS.
+  __device__ inline void load(const TG* __restrict__ X, int t) {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      if (!active(t, p)) continue;
       const bool fok = foff[p] != kInvalid;
+      unsigned m = 0;
       if (MODE == FREE4) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const uint32_t ko = offK[kb + j];
-          if (fok && ko != kInvalid) {
-            TG tmp[4];
-            load4(X + (size_t)(foff[p] + ko), tmp);
+          const bool ok = fok && kn[p][j] != kInvalid;
+          m |= ok ? (1u << j) : 0u;
+          TG tmp[4];
+          load4(X + (ok ? (size_t)(foff[p] + kn[p][j]) : (size_t)0), tmp);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[p][e][j] = tmp[e];
-          } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) v[p][e][j] = TG(0);
-          }
+          for (int e = 0; e < 4; ++e) v[p][e][j] = tmp[e];
         }
       } else if (MODE == KDIM4) {
-        const uint32_t ko = offK[kb];
-        if (fok && ko != kInvalid) {
-          load4(X + (size_t)(foff[p] + ko), v[p][0]);
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) v[p][0][j] = TG(0);
-        }
+        const bool ok = fok && kn[p][0] != kInvalid;
+        m = ok ? 0xfu : 0u;
+        load4(X + (ok ? (size_t)(foff[p] + kn[p][0]) : (size_t)0), v[p][0]);
       } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const uint32_t ko = offK[kb + j];
-          v[p][0][j] = (fok && ko != kInvalid) ? X[(size_t)(foff[p] + ko)] : TG(0);
+          const bool ok = fok && kn[p][j] != kInvalid;
+          m |= ok ? (1u << j) : 0u;
+          v[p][0][j] = X[ok ? (size_t)(foff[p] + kn[p][j]) : (size_t)0];
         }
       }
+      okm[p] = m;
     }
   }
 
   // lds points at plane 0 of this operand's stage; plane_stride in TL elements.
-  __device__ inline void store(TL* lds, int plane_stride, float scale, int t) const {
+  __device__ inline void store(TL* lds, int plane_stride, float scale, int t) {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
-      const int b = t + 256 * p;
-      if (NBLK % 256 != 0 && b >= NBLK) continue;
+      if (!active(t, p)) continue;
 #pragma unroll
-      for (int e = 0; e < VF; ++e)
+      for (int e = 0; e < VF; ++e) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[p][e][j] = ((okm[p] >> j) & 1u) ? v[p][e][j] : TG(0);
         RowWriter<P>::write(lds, plane_stride, row[p][e] * LDK + kl[p], v[p][e], scale);
+      }
     }
   }
 };
@@ -437,8 +450,14 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     sB = 448.f / b;
     descale = (a / 448.f) * (b / 448.f);
   }
-  la.load(p.A, p.offAk, 0, t);
-  lb.load(p.B, p.offBk, 0, t);
+  la.fetch_k(p.offAk, 0, t);
+  lb.fetch_k(p.offBk, 0, t);
+  la.load(p.A, t);
+  lb.load(p.B, t);
+  if (p.kTiles > 1) {
+    la.fetch_k(p.offAk, BK, t);
+    lb.fetch_k(p.offBk, BK, t);
+  }
   la.store(smem, S::PLANE_A, sA, t);
   lb.store(smem + TR::PLANES * S::PLANE_A, S::PLANE_B, sB, t);
   __syncthreads();
@@ -447,8 +466,12 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     const int stage = kt & 1;
     const bool more = (kt + 1) < p.kTiles;
     if (more) {
-      la.load(p.A, p.offAk, (kt + 1) * BK, t);
-      lb.load(p.B, p.offBk, (kt + 1) * BK, t);
+      la.load(p.A, t);
+      lb.load(p.B, t);
+      if (kt + 2 < p.kTiles) {
+        la.fetch_k(p.offAk, (kt + 2) * BK, t);
+        lb.fetch_k(p.offBk, (kt + 2) * BK, t);
+      }
     }
     const TL* As = smem + stage * S::STAGE;
     const TL* Bs = As + TR::PLANES * S::PLANE_A;
@@ -477,28 +500,36 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     __syncthreads();
   }
 
-  // Epilogue: alpha/beta update of C along its stride-one index.
+  // Epilogue: alpha/beta update of C along its stride-one index.  The C
+  // offsets of this lane's rows and columns are gathered before any store.
   const TG alpha = p.alpha, beta = p.beta;
+  uint32_t cm[S::SM], cn[S::SN][MM::NACC];
 #pragma unroll
   for (int j = 0; j < S::SM; ++j) {
     int n_l, m_l;
     MM::coord(lane, 0, n_l, m_l);
-    const int m = m0 + wm * S::WTM + j * TR::SUB + m_l;
-    const uint32_t cm = p.offCm[m];
-    const bool mok = cm != kInvalid;
+    cm[j] = p.offCm[m0 + wm * S::WTM + j * TR::SUB + m_l];
+  }
+#pragma unroll
+  for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+    for (int r = 0; r < MM::NACC; ++r) {
+      int n_l, m_l;
+      MM::coord(lane, r, n_l, m_l);
+      cn[i][r] = p.offCn[n0 + wn * S::WTN + i * TR::SUB + n_l];
+    }
+#pragma unroll
+  for (int j = 0; j < S::SM; ++j) {
 #pragma unroll
     for (int i = 0; i < S::SN; ++i) {
 #pragma unroll
       for (int r = 0; r < MM::NACC; ++r) {
-        MM::coord(lane, r, n_l, m_l);
-        const int n = n0 + wn * S::WTN + i * TR::SUB + n_l;
-        const uint32_t cn = p.offCn[n];
-        if (mok && cn != kInvalid) {
+        if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
           TG val;
           if (SecondAcc<P>::USED) val = (TG)MM::value(acc[i][j], acc2[i][j], r);
           else val = (TG)MM::value(acc[i][j], acc2[0][0], r);
           if (P == FP8) val = (TG)((float)val * descale);
-          TG* dst = p.C + (size_t)(cm + cn);
+          TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
           if (beta == TG(0)) {
             __builtin_nontemporal_store(alpha * val, dst);
           } else {
