@@ -37,6 +37,8 @@ def parse_args():
     ap.add_argument("--only", default=None, help="run one form only: f32, bw, f64, bf16x3, fp8 (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (PMC calibration)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -173,6 +175,14 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     log = (lambda s: print(s, file=sys.stderr, flush=True))
+    if args.copy_bytes:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written" % (args.copy_bytes, args.copy_bytes))
+        del src, dst
     forms = {
         "f32": ("abcd-aebf-dfce", "f32"),
         "bw": ("abc-akb-kc", "f32"),

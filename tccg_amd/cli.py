@@ -1,0 +1,262 @@
+"""The tccg command line with the gfx950 target.
+
+    python -m tccg_amd [--arch=gfx950] [--floatType=s|d] [--maxImplementations=N]
+                       [--testing] [--generateOnly] ... file.tccg
+
+Flags and their meaning follow the reference's argparse table
+(tccg/tccg.py:1253-1285) and TccgArgs defaults (tccg/tccg.py:30-59).  The
+flow follows Tccg.codeGen (tccg/tccg.py:188-430): look the contraction up in
+the tuning database, otherwise rank GETT candidates with the performance
+model, time the best --maxImplementations of them on the GPU (printing
+"<name>: X GFLOPS // estimated: Y GFLOPS" per candidate, tccg/tccg.py:1052-1056),
+store them, and emit the fastest as tccg_implementations/gett.hip + gett.hpp
++ libgett.so exporting gett(A, B, C, alpha, beta) (tccg/tccg.py:158-186).
+The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL: ..." is printed as
+the reference does (tccg/tccg.py:426); methods other than GETT are not part
+of this backend and report 0.00.
+
+Only --arch=gfx950 is provided; the reference's CPU/CUDA architectures are
+rejected.  --floatType=s runs FP32 (or, with --precision, the split-BF16 or
+FP8 forms), --floatType=d runs FP64.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import socket
+import sys
+import time
+from pathlib import Path
+
+from .frontend import TccgSyntaxError, contraction_str, fuse_indices, read_tccg_file
+from .plan import BF16X3, F32, F64, FP8, PREC_NAMES, UnsupportedContraction, candidate_plans, build_plan, estimate_seconds
+
+OKGREEN = "\033[92m"
+FAIL = "\033[91m"
+WARNING = "\033[93m"
+ENDC = "\033[0m"
+
+
+def make_parser() -> argparse.ArgumentParser:
+    p = argparse.ArgumentParser(description="Generate high-performance GPU code for a given tensor contraction.")
+    p.add_argument("filename", metavar="filename", type=str,
+                   help="filename of the file which contains the symbolic description of the tensor contraction")
+    p.add_argument("--verbose", action="store_true", help="Verbose output (useful for debugging).")
+    p.add_argument("--testing", action="store_true", help="enable testing of the generated routines.")
+    p.add_argument("--keep", action="store_true", help="keep intermediate files")
+    p.add_argument("--noBatchedGEMM", action="store_true", help="accepted for compatibility (LoG is not provided)")
+    p.add_argument("--generateOnly", action="store_true", help="only generate code (no timing on the GPU)")
+    p.add_argument("--gettOnly", action="store_true", help="Only use GETT implementation (always the case here)")
+    p.add_argument("--noGETT", action="store_true", help="Do not use GETT implementation")
+    p.add_argument("--noLoG", action="store_true", help="accepted for compatibility")
+    p.add_argument("--noGEMM", action="store_true", help="accepted for compatibility")
+    p.add_argument("--noTTGT", action="store_true", help="accepted for compatibility")
+    p.add_argument("--useTCL", action="store_true", help="not supported by this backend")
+    p.add_argument("--useDynamicMemory", action="store_true", help="accepted for compatibility")
+    p.add_argument("--timePacking", action="store_true", help="accepted for compatibility")
+    p.add_argument("--ignoreDatabase", action="store_true",
+                   help="ignore the tuning database (no lookup of an existing solution)")
+    p.add_argument("--fastMeasurements", action="store_true", help="time fewer repetitions per candidate")
+    p.add_argument("--numThreads", type=int, help="accepted for compatibility (one GPU per contraction)")
+    p.add_argument("--maxImplementations", type=int,
+                   help="maximum number of GETT candidates to evaluate (default: 16). -1 denotes infinity.")
+    p.add_argument("--maxImplementationsTTGT", type=int, help="accepted for compatibility")
+    p.add_argument("--maxImplementationsLoG", type=int, help="accepted for compatibility")
+    p.add_argument("--maxWorkspace", type=float, help="accepted for compatibility")
+    p.add_argument("--arch", metavar="arch", type=str, help="architecture: gfx950 (default)")
+    p.add_argument("--compiler", metavar="compiler", type=str, help="accepted for compatibility (hipcc is used)")
+    p.add_argument("--floatType", metavar="floatType", type=str, help="floatType can be either 's' or 'd'.")
+    p.add_argument("--precision", choices=["fp32", "split-bf16", "fp8"], default="fp32",
+                   help="arithmetic of --floatType=s: plain FP32 MFMA, three-way BF16 split, or FP8 e4m3")
+    p.add_argument("--affinity", metavar="affinity", type=str, help="accepted for compatibility")
+    p.add_argument("--output", type=str, default=None,
+                   help="output directory (default: ./tccg_implementations)")
+    p.add_argument("--database", type=str, default=None, help="tuning database path (default: $TCCG_DB or "
+                                                               "~/.cache/tccg_amd/tccg.db)")
+    return p
+
+
+def settings_str(args, prec: int) -> str:
+    rows = [("Version", "tccg_amd 0.1.0 (gfx950)"), ("inputfile", args.filename),
+            ("maxImplementations", str(args.maxImplementations)), ("compiler", "hipcc"),
+            ("architecture", args.arch), ("floatType", {F64: "double"}.get(prec, "float")),
+            ("precision", PREC_NAMES[prec]), ("hostname", socket.gethostname())]
+    out = "-----------------SETTINGS-----------------\n"
+    out += "".join(k.ljust(20) + v + "\n" for k, v in rows)
+    return out + "------------------------------------------"
+
+
+def candidate_name(plan) -> str:
+    """Reference-style candidate name (tccg/gett.py:578-579); decodable by the
+    reference's decodeName (tccg/gett.py:561-576)."""
+    from .plan import TILES
+    pb = plan.problem
+    blocked = 0 if plan.lin_m.is_plain() else 1
+    wtm, wtn = plan.BM // 2, plan.BN // 2
+    sub = 16 if plan.prec in (F64, FP8) else 32
+    return "gett_var%d_mc%d_nc%d_kc%d_mc1%d_nc1%d_mr%d_nr%d_%s_%s_%s" % (
+        blocked, plan.BM, plan.BN, plan.BK, wtm, wtn, sub, sub, pb.opA.index_str(), pb.opB.index_str(),
+        pb.C.index_str())
+
+
+def _precision(args) -> int:
+    ft = args.floatType or "s"
+    if ft == "d":
+        return F64
+    if ft != "s":
+        raise SystemExit(FAIL + "ERROR: floatType %s is not supported yet." % ft + ENDC)
+    return {"fp32": F32, "split-bf16": BF16X3, "fp8": FP8}[args.precision]
+
+
+def main(argv=None) -> int:
+    args = make_parser().parse_args(argv)
+    args.arch = args.arch or "gfx950"
+    if args.maxImplementations is None:
+        args.maxImplementations = 16
+    elif args.maxImplementations == -1:
+        args.maxImplementations = 1000000000
+    if args.arch != "gfx950":
+        print(FAIL + "[GETT] Error: architecture %s is not provided by this backend (use --arch=gfx950)"
+              % args.arch + ENDC, file=sys.stderr)
+        return 1
+    if args.noGETT:
+        print(FAIL + "ERROR: --noGETT leaves no method: this backend provides GETT only" + ENDC, file=sys.stderr)
+        return 1
+    try:
+        prec = _precision(args)
+    except SystemExit as e:
+        print(e, file=sys.stderr)
+        return 1
+    print(settings_str(args, prec))
+    path = args.filename if os.path.isabs(args.filename) else os.path.join(os.getcwd(), args.filename)
+    try:
+        text = Path(path).read_text() if os.path.isfile(path) else None
+        if text is None:
+            raise TccgSyntaxError("File: %s does not exist." % path)
+        from .contraction import Contraction
+        contraction = Contraction.from_tccg(text)
+    except (TccgSyntaxError, UnsupportedContraction) as e:
+        print(FAIL + str(e) + ENDC, file=sys.stderr)
+        return 1
+    c0 = contraction_str(contraction.C0, contraction.A0, contraction.B0, contraction.alpha, contraction.beta)
+    c1 = str(contraction)
+    if c0 != c1:
+        print("Contration before fusing indices:")
+        print(c0)
+        print("Contration after fusing indices:")
+    print(c1)
+    pb = contraction.problem
+    print("m: %d\nn: %d\nk: %d" % (pb.M, pb.N, pb.K))
+
+    out_dir = Path(args.output) if args.output else Path(os.getcwd()) / "tccg_implementations"
+    plans = candidate_plans(pb, prec)[: max(1, args.maxImplementations)]
+    print("Total amount of GETT implementations: %d" % len(plans))
+    chosen = plans[0]
+    best_gflops = -1.0
+    ref_gflops = -1.0
+    gpu = False
+    if not args.generateOnly:
+        try:
+            import torch
+            gpu = torch.cuda.is_available()
+        except Exception:
+            gpu = False
+        if not gpu:
+            print(WARNING + "WARNING: no GPU visible; emitting the model's best candidate without timing" + ENDC)
+    if gpu:
+        from .tuning_db import TuningDB
+        db = TuningDB(args.database)
+        device = "gfx950"
+        hit = None if args.ignoreDatabase else db.fastest(contraction, device, PREC_NAMES[prec], contraction.beta)
+        if hit is not None:
+            name, tile, blocked, gf = hit
+            chosen = build_plan(pb, prec, tile, m_blocked=blocked)
+            best_gflops = gf
+            print(OKGREEN + "[database] reusing %s (%.2f GFLOPS)" % (name, gf) + ENDC)
+        else:
+            best_gflops, chosen, ref_gflops = _time_candidates(args, contraction, prec, plans, db, device)
+            if chosen is None:
+                return 1
+        db.close()
+    print(OKGREEN + "[generate] emitting %s" % candidate_name(chosen) + ENDC)
+    try:
+        from . import codegen
+        res = codegen.generate(chosen, text, out_dir, "gett")
+    except Exception as e:
+        print(FAIL + "[TCC Error] compilation failed: %s" % e + ENDC, file=sys.stderr)
+        return 1
+    print("The generated code is available at:", res["source"])
+    print("Best gett-based implementation (%s) attained: %.2f GFLOPS/s" % (candidate_name(chosen), max(best_gflops, 0)))
+    print("Best Loop/TTGT/GETT/reference/GEMM/TCL: %.2f / %.2f / %.2f / %.2f / %.2f / %.2f GFLOPS"
+          % (0.0, 0.0, max(best_gflops, 0.0), ref_gflops, 0.0, -1.0))
+    return 0
+
+
+def _time_candidates(args, contraction, prec, plans, db, device):
+    import numpy as np
+    import torch
+    from .autotune import time_plan
+    from .runtime import DevicePlan
+    dt = torch.float64 if prec == F64 else torch.float32
+    sa, sb, sc = contraction.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C = C0.clone()
+    ref = None
+    ref_gflops = -1.0
+    if args.testing:
+        t0 = time.perf_counter()
+        ref = _fp64_reference(contraction, A, B, C0)
+        ref_gflops = contraction.flops() / (time.perf_counter() - t0) / 1e9
+    print(OKGREEN + "[running] Measuring the runtime of all variants ..." + ENDC)
+    mid = db.measurement_id(contraction, device, PREC_NAMES[prec], contraction.beta, create=True)
+    best = (-1.0, None)
+    reps = 2 if args.fastMeasurements else 5
+    tol = {F32: 1e-5, F64: 1e-13, BF16X3: 1e-5, FP8: 8e-2}[prec]
+    failed = 0
+    for p in plans:
+        dp = DevicePlan(p)
+        name = candidate_name(p)
+        if ref is not None:
+            C.copy_(C0)
+            dp.run(A, B, C, contraction.alpha, contraction.beta)
+            torch.cuda.synchronize()
+            err = float((C.double() - ref).norm() / ref.norm())
+            if not err <= tol:
+                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+                failed += 1
+                continue
+        C.copy_(C0)
+        t = time_plan(dp, A, B, C, contraction.alpha, contraction.beta, reps=reps)
+        gf = contraction.flops() / t / 1e9
+        est = contraction.flops() / estimate_seconds(p, contraction.beta) / 1e9
+        print("%s: %.2f GFLOPS // estimated: %.2f GFLOPS" % (name, gf, est))
+        db.insert_gett(mid, name, p.tile, not p.lin_m.is_plain(), gf, est)
+        if gf > best[0]:
+            best = (gf, p)
+    if failed:
+        print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
+        return best[0], None, ref_gflops
+    return best[0], best[1], ref_gflops
+
+
+def _fp64_reference(contraction, A, B, C0):
+    import string
+    import torch
+    letters = {}
+    pool = iter(string.ascii_letters)
+    for lab in [i.label for i in contraction.C0.indices + contraction.A0.indices + contraction.B0.indices]:
+        letters.setdefault(lab, next(pool))
+    rev = lambda T: "".join(letters[i.label] for i in reversed(T.indices))
+    eq = "%s,%s->%s" % (rev(contraction.A0), rev(contraction.B0), rev(contraction.C0))
+    out = torch.einsum(eq, A.double().contiguous(), B.double().contiguous()) * contraction.alpha
+    if contraction.beta != 0:
+        out = out + contraction.beta * C0.double()
+    torch.cuda.synchronize()
+    return out
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -36,6 +36,15 @@ class Contraction:
     def __init__(self, C: Tensor, A: Tensor, B: Tensor, alpha: float = 1.0, beta: float = 0.0,
                  fuse: bool = True):
         import copy
+        from .algebra import get_loop_indices
+        from .plan import UnsupportedContraction
+        # Batch indices are rejected before fusion: fusing a run that contains
+        # one would leave it in a single tensor (the reference fuses first,
+        # tccg/tccg.py:104, and only then checks, tccg/gett.py:42-45).
+        loop = get_loop_indices(A, B, C)
+        if loop:
+            raise UnsupportedContraction("loop indices are not yet supported by GETT: %s"
+                                         % ",".join(i.label for i in loop))
         # the caller-visible (unfused) tensors define the expected element counts
         self.C0, self.A0, self.B0 = copy.deepcopy(C), copy.deepcopy(A), copy.deepcopy(B)
         self.C, self.A, self.B = copy.deepcopy(C), copy.deepcopy(A), copy.deepcopy(B)

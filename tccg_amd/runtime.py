@@ -158,3 +158,38 @@ class DevicePlan:
         rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
         if rc != 0:
             raise GettRuntimeError("GETT launch failed: %s" % last_error())
+
+
+class GeneratedGett:
+    """A per-contraction library emitted by tccg_amd.codegen (exports gett,
+    gett_stream, gett_info, gett_contraction, gett_last_error)."""
+
+    INFO_FIELDS = ("M", "N", "K", "prec", "tile", "BM", "BN", "BK", "modeA", "modeB")
+
+    def __init__(self, path):
+        self.path = Path(path)
+        self.lib = ctypes.CDLL(str(self.path))
+        buf = (ctypes.c_int64 * 10)()
+        self.lib.gett_info.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        n = self.lib.gett_info(buf, 10)
+        self.info = dict(zip(self.INFO_FIELDS, list(buf)[:n]))
+        T = ctypes.c_double if self.info["prec"] == F64 else ctypes.c_float
+        self.lib.gett_stream.argtypes = [ctypes.c_void_p] * 3 + [T, T, ctypes.c_void_p]
+        self.lib.gett_stream.restype = ctypes.c_int
+        self.lib.gett.argtypes = [ctypes.c_void_p] * 3 + [T, T]
+        self.lib.gett.restype = None
+        self.lib.gett_contraction.restype = ctypes.c_char_p
+        self.lib.gett_last_error.restype = ctypes.c_char_p
+
+    def contraction(self) -> str:
+        return self.lib.gett_contraction().decode()
+
+    def __call__(self, A, B, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        torch = _check_cuda()
+        for name, t in (("A", A), ("B", B), ("C", C)):
+            if t.device.type != "cuda" or not t.is_contiguous():
+                raise GettRuntimeError("%s must be a contiguous GPU tensor" % name)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = self.lib.gett_stream(A.data_ptr(), B.data_ptr(), C.data_ptr(), alpha, beta, ctypes.c_void_p(s.cuda_stream))
+        if rc != 0:
+            raise GettRuntimeError("generated gett failed: %s" % self.lib.gett_last_error().decode())

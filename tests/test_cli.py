@@ -1,0 +1,97 @@
+"""The tccg command line and the gfx950 code generator (CPU: generate-only)."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from tccg_amd import Contraction, codegen
+from tccg_amd.cli import candidate_name, main
+from tccg_amd.plan import F32, F64, FP8
+from tccg_amd.workloads import WORKLOADS
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def decode_name(variant):
+    """The reference's Gett.decodeName (tccg/gett.py:561-576), restated."""
+    tokens = variant.split("_")
+    return (int(tokens[1][3:]), int(tokens[2][2:]), int(tokens[3][2:]), int(tokens[4][2:]), int(tokens[5][3:]),
+            int(tokens[6][3:]), int(tokens[7][2:]), int(tokens[8][2:]), "_".join(tokens[9:]))
+
+
+def test_candidate_names_decode():
+    c = Contraction.from_tccg(WORKLOADS["abcd-aebf-dfce"])
+    p = c.plan(F32)
+    var, mc, nc, kc, mc1, nc1, mr, nr, idx = decode_name(candidate_name(p))
+    assert (mc, nc, kc) == (p.BM, p.BN, p.BK)
+    assert idx == "aebf_dfce_abcd"
+
+
+def test_cli_generate_only(tmp_path):
+    f = tmp_path / "example.tccg"
+    f.write_text(WORKLOADS["example"])
+    out = tmp_path / "impl"
+    rc = main(["--arch=gfx950", "--floatType=s", "--generateOnly", "--output", str(out), str(f)])
+    assert rc == 0
+    assert (out / "gett.hip").exists() and (out / "gett.hpp").exists() and (out / "libgett.so").exists()
+    src = (out / "gett.hip").read_text()
+    assert 'extern "C"' in src and "void gett(float* A, float* B, float* C, const float alpha, const float beta)" in src
+    syms = subprocess.run(["nm", "-D", str(out / "libgett.so")], stdout=subprocess.PIPE, text=True).stdout
+    for s in ("gett", "gett_stream", "gett_info", "gett_contraction", "gett_last_error"):
+        assert (" T %s\n" % s) in syms
+
+
+def test_cli_double_header(tmp_path):
+    f = tmp_path / "c.tccg"
+    f.write_text("C[a,b] = A[a,k] * B[k,b]\na=8\nb=8\nk=8\n")
+    rc = main(["--floatType=d", "--generateOnly", "--output", str(tmp_path / "o"), str(f)])
+    assert rc == 0
+    assert "void gett(double* A, double* B, double* C, const double alpha, const double beta);" in \
+        (tmp_path / "o" / "gett.hpp").read_text()
+
+
+@pytest.mark.parametrize("argv,err", [
+    (["--arch=avx2"], "architecture"),
+    (["--floatType=z"], "floatType"),
+    (["--noGETT"], "GETT only"),
+])
+def test_cli_rejects(tmp_path, capsys, argv, err):
+    f = tmp_path / "e.tccg"
+    f.write_text(WORKLOADS["example"])
+    assert main(argv + ["--generateOnly", str(f)]) != 0
+    assert err in capsys.readouterr().err
+
+
+def test_cli_syntax_error_exit_status(tmp_path):
+    f = tmp_path / "bad.tccg"
+    f.write_text("C[a,b] += A[a,k] * B[k,b]\na=2\nb=3\nk=4\n")
+    proc = subprocess.run([sys.executable, "-m", "tccg_amd", "--generateOnly", str(f)], cwd=ROOT,
+                          stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert proc.returncode != 0 and "+=" in proc.stderr
+
+
+def test_cli_missing_file(tmp_path, capsys):
+    assert main(["--generateOnly", str(tmp_path / "nope.tccg")]) != 0
+
+
+def test_cli_batch_index_rejected(tmp_path, capsys):
+    f = tmp_path / "l.tccg"
+    f.write_text("C[a,b,l] = A[a,k,l] * B[k,b,l]\na=2\nb=3\nk=4\nl=5\n")
+    assert main(["--generateOnly", str(f)]) != 0
+    assert "loop indices" in capsys.readouterr().err
+
+
+def test_emit_source_tables_and_swap():
+    # B holds C's stride-one index: the generated entry swaps the operands internally
+    c = Contraction.from_labels(["b", "a"], ["a", "k"], ["k", "b"], dict(a=8, b=12, k=4))
+    p = c.plan(F32)
+    src = codegen.emit_source(p, "C[b,a] = A[a,k] * B[k,b]")
+    assert "const float* opA = 1 ? B : A;" in src
+    assert "kOffAm[%d]" % p.Mp in src and "kOffCn[%d]" % p.Np in src
+
+
+def test_emit_fp8_reduces_amax():
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    src = codegen.emit_source(c.plan(FP8), WORKLOADS["example"])
+    assert "amax_reduce" in src and "t->amax" in src

@@ -98,3 +98,26 @@ def test_fused_indices_run():
     c = Contraction.from_labels(["a", "b", "c"], ["a", "b", "k"], ["k", "c"], dict(a=12, b=10, c=20, k=9))
     assert len(c.C.indices) == 2
     check(c, F32, seed=7)
+
+
+def _golden_cases():
+    import glob
+    import os
+    return sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", _golden_cases(), ids=lambda p: p.split("/")[-1])
+@pytest.mark.parametrize("prec", [F32, BF16X3])
+def test_golden_fixtures(path, prec):
+    """GPU result against the committed FP64 fixture (inputs stored as FP32)."""
+    import torch
+    from oracle.einsum_ref import rel_frobenius
+    z = np.load(path, allow_pickle=False)
+    sizes = dict(zip([str(x) for x in z["size_labels"]], [int(x) for x in z["size_values"]]))
+    c = Contraction.from_labels([str(x) for x in z["c"]], [str(x) for x in z["a"]], [str(x) for x in z["b"]],
+                                sizes, float(z["alpha"]), float(z["beta"]))
+    dev = torch.device("cuda", 0)
+    tA, tB, tC = (torch.from_numpy(np.ascontiguousarray(z[k])).to(dev) for k in ("A", "B", "C_in"))
+    c(tA, tB, tC, prec=prec)
+    torch.cuda.synchronize()
+    assert rel_frobenius(tC.cpu().numpy(), z["C_out"]) < 1e-6
