@@ -41,7 +41,8 @@ enum tccg_prec {
   TCCG_F32 = 0,    /* FP32 MFMA, FP32 accumulate (--floatType=s)            */
   TCCG_F64 = 1,    /* FP64 MFMA (--floatType=d)                             */
   TCCG_BF16X3 = 2, /* FP32 operands as three BF16 parts, 6 BF16 MFMA products */
-  TCCG_FP8 = 3     /* FP32 operands scaled per tensor to OCP e4m3, FP8 MFMA  */
+  TCCG_FP8 = 3,    /* FP32 operands scaled per tensor to OCP e4m3, FP8 MFMA  */
+  TCCG_BF16 = 4    /* FP32 operands rounded to BF16, one BF16 MFMA product  */
 };
 
 /* How an operand tile is gathered (see tccg_amd/csrc/gett_core.hpp). */
@@ -96,6 +97,51 @@ int tccg_gett_bf16x3(const float* A, const float* B, float* C, float alpha, floa
                      const tccg_gett_plan* plan, void* stream);
 int tccg_gett_fp8(const float* A, const float* B, float* C, float alpha, float beta,
                   const tccg_gett_plan* plan, void* stream);
+int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float beta,
+                   const tccg_gett_plan* plan, void* stream);
+
+/*
+ * Batched GETT: many independent FP32 contractions in ONE persistent launch.
+ * No counterpart exists in the reference (its LoG method loops over BLAS
+ * calls, tccg/gemm_loop.py:76-314); the entry point serves callers with
+ * hundreds of small, uneven contractions and is safe to capture in a HIP
+ * graph (no allocation, no memset, no host synchronisation).
+ *
+ * `problems` (device array) describes each contraction with the same tables
+ * as tccg_gett_plan, its internal operands (A holds C's stride-one index),
+ * tile counts and a variant id from tccg_batch_variant().  `work` (device
+ * array) lists the C tiles to compute, (problem, tm, tn); a grid of `grid`
+ * workgroups walks it with stride `grid`.  Every tile of every problem must
+ * appear exactly once.
+ */
+typedef struct tccg_batch_problem {
+  const float* A;
+  const float* B;
+  float* C;
+  const uint32_t* offAm;
+  const uint32_t* offAk;
+  const uint32_t* offBn;
+  const uint32_t* offBk;
+  const uint32_t* offCm;
+  const uint32_t* offCn;
+  const int32_t* ordA;
+  const int32_t* ordB;
+  int32_t tilesM, tilesN, kTiles;
+  int32_t variant;
+  float alpha, beta;
+} tccg_batch_problem;
+
+typedef struct tccg_batch_work {
+  int32_t problem, tm, tn, pad;
+} tccg_batch_work;
+
+/* Number of batch variants and the tile shape / load modes of variant `id`. */
+int tccg_batch_num_variants(void);
+int tccg_batch_variant(int id, int* BM, int* BN, int* BK, int* modeA, int* modeB);
+
+/* Batched plan-driven FP32 GETT (one launch on `stream`). */
+int tccg_gett_batch_f32(const tccg_batch_problem* problems, const tccg_batch_work* work, int nwork, int grid,
+                        void* stream);
 
 /* out[0] = max_i |x[i]| (device float); used for the FP8 per-tensor scales. */
 int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream);

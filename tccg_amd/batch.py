@@ -1,0 +1,191 @@
+"""Batched GETT: hundreds of uneven FP32 contractions in one persistent launch.
+
+Each contraction gets a GETT plan (tables as for a single launch); the plans'
+tables are concatenated into one device buffer, a problem table
+(tccg_batch_problem, include/tccg_gfx950.h) points into it, and a work list
+enumerates every C tile of every problem, longest k-loop first (LPT order), so
+that the persistent grid's round-robin walk balances the load.  The whole
+batch is one kernel launch, which can be captured in a HIP graph.
+
+Tile shapes are chosen per index pattern: `autotune_patterns()` times each
+pattern's problems with each batch tile shape (the GPU analogue of the
+reference's timed search, tccg/tccg.py:278-430) and keeps the faster one.
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import defaultdict
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .contraction import Contraction
+from .plan import F32, TILES, build_plan
+
+# batch shape id -> tile id of TILES[F32] with the same (BM, BN, BK)
+BATCH_SHAPES = {0: TILES[F32].index((64, 64, 32)), 1: TILES[F32].index((128, 128, 32))}
+
+
+class _CProblem(ctypes.Structure):
+    _fields_ = [("A", ctypes.c_void_p), ("B", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("offAm", ctypes.c_void_p), ("offAk", ctypes.c_void_p), ("offBn", ctypes.c_void_p),
+                ("offBk", ctypes.c_void_p), ("offCm", ctypes.c_void_p), ("offCn", ctypes.c_void_p),
+                ("ordA", ctypes.c_void_p), ("ordB", ctypes.c_void_p),
+                ("tilesM", ctypes.c_int32), ("tilesN", ctypes.c_int32), ("kTiles", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("alpha", ctypes.c_float), ("beta", ctypes.c_float)]
+
+
+def pattern_key(c: Contraction) -> str:
+    return "%s=%s*%s" % (c.C0.index_str(), c.A0.index_str(), c.B0.index_str())
+
+
+class GettBatch:
+    """Fixed set of contractions bound to fixed GPU tensors, launched together."""
+
+    def __init__(self, contractions: Sequence[Contraction], shapes: Optional[Dict[str, int]] = None,
+                 alphas: Optional[Sequence[float]] = None, betas: Optional[Sequence[float]] = None):
+        self.contractions = list(contractions)
+        self.shapes = dict(shapes or {})
+        self.alphas = list(alphas) if alphas is not None else [c.alpha for c in self.contractions]
+        self.betas = list(betas) if betas is not None else [c.beta for c in self.contractions]
+        self.plans = []
+        for c in self.contractions:
+            sh = self.shapes.get(pattern_key(c), self._default_shape(c))
+            self.plans.append(build_plan(c.problem, F32, BATCH_SHAPES[sh]))
+        self._bound = None
+
+    @staticmethod
+    def _default_shape(c: Contraction) -> int:
+        return 1 if c.problem.M * c.problem.N >= 256 * 256 else 0
+
+    def flops(self) -> float:
+        return sum(c.flops(b) for c, b in zip(self.contractions, self.betas))
+
+    def bind(self, As, Bs, Cs, device=None, grid: Optional[int] = None):
+        """Build the device problem/work tables for these tensors (their
+        pointers are baked in; reuse the same tensors for every launch)."""
+        import torch
+        from .runtime import load_library
+        lib = load_library()
+        dev = device or As[0].device
+        # all tables of all problems in one int32 buffer
+        chunks, offsets, pos = [], [], 0
+        for p in self.plans:
+            offs = []
+            for arr in (p.offAm, p.offAk, p.offBn, p.offBk, p.offCm, p.offCn, p.ordA, p.ordB):
+                a = np.ascontiguousarray(arr).view(np.int32)
+                offs.append(pos)
+                chunks.append(a)
+                pos += a.size
+                pad = (-pos) % 4   # keep every table 16-byte aligned
+                if pad:
+                    chunks.append(np.zeros(pad, np.int32))
+                    pos += pad
+            offsets.append(offs)
+        tables = torch.from_numpy(np.concatenate(chunks)).to(dev)
+        base = tables.data_ptr()
+        probs = (_CProblem * len(self.plans))()
+        work = []
+        for i, (p, c, A, B, C) in enumerate(zip(self.plans, self.contractions, As, Bs, Cs)):
+            for name, t, T in (("A", A, c.A0), ("B", B, c.B0), ("C", C, c.C0)):
+                if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size():
+                    raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements"
+                                     % (i, name, T.size()))
+            opA, opB = (B, A) if p.problem.swapped else (A, B)
+            q = probs[i]
+            q.A, q.B, q.C = opA.data_ptr(), opB.data_ptr(), C.data_ptr()
+            (q.offAm, q.offAk, q.offBn, q.offBk, q.offCm, q.offCn, q.ordA, q.ordB) = \
+                [base + 4 * o for o in offsets[i]]
+            q.tilesM, q.tilesN, q.kTiles = p.Mp // p.BM, p.Np // p.BN, p.Kp // p.BK
+            sh = [k for k, v in BATCH_SHAPES.items() if v == p.tile][0]
+            q.variant = sh * 16 + p.modeA * 4 + p.modeB
+            q.alpha, q.beta = self.alphas[i], self.betas[i]
+            cost = p.BM * p.BN * q.kTiles
+            for tm in range(q.tilesM):
+                for tn in range(q.tilesN):
+                    work.append((cost, i, tm, tn))
+        # longest tiles first: the round-robin walk then approximates LPT
+        work.sort(key=lambda w: -w[0])
+        wl = np.array([[w[1], w[2], w[3], 0] for w in work], dtype=np.int32)
+        pbytes = np.frombuffer(bytes(probs), dtype=np.uint8)
+        dprobs = torch.from_numpy(pbytes.copy()).to(dev)
+        dwork = torch.from_numpy(wl).to(dev)
+        if grid is None:
+            grid = min(len(work), 256 * 2)
+        self._bound = dict(tables=tables, probs=dprobs, work=dwork, nwork=len(work), grid=int(grid),
+                           tensors=(list(As), list(Bs), list(Cs)), lib=lib)
+        return self
+
+    def launch(self, stream=None) -> None:
+        import torch
+        from .runtime import GettRuntimeError
+        b = self._bound
+        if b is None:
+            raise RuntimeError("bind() the batch to tensors first")
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rc = b["lib"].tccg_gett_batch_f32(ctypes.c_void_p(b["probs"].data_ptr()), ctypes.c_void_p(b["work"].data_ptr()),
+                                          b["nwork"], b["grid"], ctypes.c_void_p(s.cuda_stream))
+        if rc != 0:
+            raise GettRuntimeError("batched GETT failed: %s" % b["lib"].tccg_last_error().decode())
+
+    def capture(self):
+        """A torch.cuda.CUDAGraph (HIP graph) holding the one batched launch."""
+        import torch
+        self.launch()              # first launch outside capture sets kernel attributes
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.launch()
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+
+def random_batch(n: int, seed: int, patterns=None, lo: int = 8, hi: int = 96, step: int = 8):
+    """n contractions drawn uniformly from `patterns`, every extent uniform over
+    multiples of `step` in [lo, hi] (numpy default_rng(seed))."""
+    from .workloads import BATCH_PATTERNS
+    patterns = patterns or BATCH_PATTERNS
+    rng = np.random.default_rng(seed)
+    choices = np.arange(lo, hi + 1, step)
+    out = []
+    for _ in range(n):
+        c_l, a_l, b_l = patterns[int(rng.integers(len(patterns)))]
+        labels = sorted(set(c_l + a_l + b_l))
+        sizes = {l: int(rng.choice(choices)) for l in labels}
+        out.append(Contraction.from_labels(c_l, a_l, b_l, sizes))
+    return out
+
+
+def autotune_patterns(contractions: Sequence[Contraction], As, Bs, Cs, reps: int = 3, verbose: bool = False):
+    """Per index pattern, the batch tile shape with the shortest batched launch
+    over that pattern's problems."""
+    import torch
+    groups = defaultdict(list)
+    for i, c in enumerate(contractions):
+        groups[pattern_key(c)].append(i)
+    best = {}
+    for key, idx in groups.items():
+        times = {}
+        for sh in BATCH_SHAPES:
+            b = GettBatch([contractions[i] for i in idx], {key: sh}).bind(
+                [As[i] for i in idx], [Bs[i] for i in idx], [Cs[i] for i in idx])
+            b.launch()
+            torch.cuda.synchronize()
+            t = float("inf")
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                b.launch()
+                e1.record()
+                e1.synchronize()
+                t = min(t, e0.elapsed_time(e1) * 1e-3)
+            times[sh] = t
+        best[key] = min(times, key=times.get)
+        if verbose:
+            import sys
+            print("  pattern %-14s %3d problems: %s -> shape %d" % (
+                key, len(idx), {k: "%.1f us" % (v * 1e6) for k, v in times.items()}, best[key]), file=sys.stderr)
+    return best

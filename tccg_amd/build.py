@@ -32,7 +32,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_UNITS = ["gett_common.hip", "gett_f32.hip", "gett_f64.hip", "gett_bf16x3.hip", "gett_fp8.hip"]
+LIB_UNITS = ["gett_common.hip", "gett_f32.hip", "gett_f64.hip", "gett_bf16x3.hip", "gett_fp8.hip", "gett_bf16.hip",
+             "gett_batch.hip"]
 LIB_HEADERS = ["gett_core.hpp", "gett_launch.hpp"]
 
 

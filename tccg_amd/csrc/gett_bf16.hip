@@ -1,0 +1,27 @@
+// BF16 instantiations of the plan-driven GETT library.
+#include "gett_launch.hpp"
+
+namespace tccg_lib {
+
+#define TCCG_TILE_CASE(ARR, ID)                                                                    \
+  case ID:                                                                                        \
+    return launch_modes<P, ARR[ID].BM, ARR[ID].BN, ARR[ID].BK, ARR[ID].WM, ARR[ID].WN>(ma, mb, prm, s);
+
+template <>
+int dispatch_tile<tccg::BF16>(int tile, int ma, int mb, const tccg::GettParams<float>& prm, hipStream_t s) {
+  constexpr int P = tccg::BF16;
+  switch (tile) {
+    TCCG_TILE_CASE(kBf1Tiles, 0)
+    TCCG_TILE_CASE(kBf1Tiles, 1)
+    TCCG_TILE_CASE(kBf1Tiles, 2)
+    default: return fail(1, "invalid tile id");
+  }
+}
+#undef TCCG_TILE_CASE
+
+}  // namespace tccg_lib
+
+extern "C" int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float beta,
+                              const tccg_gett_plan* plan, void* stream) {
+  return tccg_lib::run_gett<tccg::BF16>(A, B, C, alpha, beta, plan, stream);
+}

@@ -23,8 +23,9 @@
 //   F64    : v_mfma_f64_16x16x4_f64 (output lane l, reg r -> row l/16+4r).
 //   BF16X3 : FP32 operands split on the fly into three BF16 parts
 //            a = a0 + a1 + a2 (round-to-nearest-even at each step, exact),
-//            six products a0b0 (own accumulator) and a2b0+a1b1+a0b2+a1b0+a0b1
-//            (second accumulator) on v_mfma_f32_32x32x16_bf16.
+//            six products a2b0+a1b1+a0b2+a1b0+a0b1+a0b0 (small terms first)
+//            into one FP32 accumulator on v_mfma_f32_32x32x16_bf16.
+//   BF16   : FP32 operands rounded to BF16 (one product, FP32 accumulate).
 //   FP8    : FP32 operands scaled per tensor and converted to OCP e4m3 on the
 //            fly, v_mfma_scale_f32_16x16x128_f8f6f4 with unit block scales,
 //            FP32 accumulate, descaled in the epilogue.
@@ -43,7 +44,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 
-enum Prec : int { F32 = 0, F64 = 1, BF16X3 = 2, FP8 = 3 };
+enum Prec : int { F32 = 0, F64 = 1, BF16X3 = 2, FP8 = 3, BF16 = 4 };
 
 // How one operand's tile is gathered from HBM.  The host picks the mode from
 // where the operand's stride-one index sits:
@@ -91,6 +92,10 @@ template <> struct PrecTraits<BF16X3> {
   using TG = float; using TL = __bf16;
   static constexpr int PLANES = 3, SUB = 32, KF = 16, PAD = 8, ACC = 16;
 };
+template <> struct PrecTraits<BF16> {
+  using TG = float; using TL = __bf16;
+  static constexpr int PLANES = 1, SUB = 32, KF = 16, PAD = 8, ACC = 16;
+};
 template <> struct PrecTraits<FP8> {
   using TG = float; using TL = uint8_t;
   static constexpr int PLANES = 1, SUB = 16, KF = 128, PAD = 16, ACC = 4;
@@ -134,6 +139,14 @@ template <> struct RowWriter<BF16X3> {
     *reinterpret_cast<bf16x4*>(lds + idx) = p0;
     *reinterpret_cast<bf16x4*>(lds + plane_stride + idx) = p1;
     *reinterpret_cast<bf16x4*>(lds + 2 * plane_stride + idx) = p2;
+  }
+};
+template <> struct RowWriter<BF16> {
+  __device__ static inline void write(__bf16* lds, int, int idx, const float* v, float) {
+    bf16x4 p0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p0[j] = (__bf16)v[j];
+    *reinterpret_cast<bf16x4*>(lds + idx) = p0;
   }
 };
 template <> struct RowWriter<FP8> {
@@ -263,107 +276,107 @@ struct TileLoader {
 };
 
 // ---------------------------------------------------------------------------
-// MFMA micro-tile policies.  Fragments are read from k-contiguous LDS rows.
+// MFMA micro-tile policies.  Fragments are read from k-contiguous LDS rows;
+// `b` is the n-operand (MFMA rows), `a` the m-operand (MFMA columns).
 // ---------------------------------------------------------------------------
 template <int P> struct Mma;
 
 template <> struct Mma<F32> {
   using Frag = f32x4;
   using Acc = f32x16;
-  static constexpr int SUB = 32, KF = 8;
-  __device__ static inline Frag read(const float* lds, int, int row0, int kk, int LDK, int lane, int) {
+  static constexpr int NACC = 16;
+  __device__ static inline Frag read(const float* lds, int, int row0, int kk, int LDK, int lane) {
     return *reinterpret_cast<const f32x4*>(lds + (row0 + (lane & 31)) * LDK + kk + 4 * (lane >> 5));
   }
-  __device__ static inline void zero(Acc& a, Acc&) { a = Acc{}; }
-  __device__ static inline void mma(Acc& acc, Acc&, const Frag& b, const Frag& a) {
+  __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b[s], a[s], acc, 0, 0, 0);
   }
-  __device__ static inline float value(const Acc& a, const Acc&, int r) { return a[r]; }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     m = lane & 31;
   }
-  static constexpr int NACC = 16;
 };
 
 template <> struct Mma<F64> {
   using Frag = f64x2;
   using Acc = f64x4;
-  static constexpr int SUB = 16, KF = 8;
-  __device__ static inline Frag read(const double* lds, int, int row0, int kk, int LDK, int lane, int) {
+  static constexpr int NACC = 4;
+  __device__ static inline Frag read(const double* lds, int, int row0, int kk, int LDK, int lane) {
     return *reinterpret_cast<const f64x2*>(lds + (row0 + (lane & 15)) * LDK + kk + 2 * (lane >> 4));
   }
-  __device__ static inline void zero(Acc& a, Acc&) { a = Acc{}; }
-  __device__ static inline void mma(Acc& acc, Acc&, const Frag& b, const Frag& a) {
+  __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(b[s], a[s], acc, 0, 0, 0);
   }
-  __device__ static inline double value(const Acc& a, const Acc&, int r) { return a[r]; }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = (lane >> 4) + 4 * r;
     m = lane & 15;
   }
-  static constexpr int NACC = 4;
 };
 
 struct Bf16x3Frag { bf16x8 p[3]; };
 template <> struct Mma<BF16X3> {
   using Frag = Bf16x3Frag;
   using Acc = f32x16;
-  static constexpr int SUB = 32, KF = 16;
-  __device__ static inline Frag read(const __bf16* lds, int plane_stride, int row0, int kk, int LDK, int lane, int) {
+  static constexpr int NACC = 16;
+  __device__ static inline Frag read(const __bf16* lds, int plane_stride, int row0, int kk, int LDK, int lane) {
     Frag f;
     const int idx = (row0 + (lane & 31)) * LDK + kk + 8 * (lane >> 5);
 #pragma unroll
     for (int q = 0; q < 3; ++q) f.p[q] = *reinterpret_cast<const bf16x8*>(lds + q * plane_stride + idx);
     return f;
   }
-  __device__ static inline void zero(Acc& hi, Acc& lo) { hi = Acc{}; lo = Acc{}; }
-  // b is the n-operand (MFMA rows), a the m-operand (MFMA columns).
-  __device__ static inline void mma(Acc& hi, Acc& lo, const Frag& b, const Frag& a) {
-    lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[2], lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[1], lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[2], a.p[0], lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[1], lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[0], lo, 0, 0, 0);
-    hi = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[0], hi, 0, 0, 0);
+  __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[2], a.p[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[0], acc, 0, 0, 0);
   }
-  __device__ static inline float value(const Acc& hi, const Acc& lo, int r) { return hi[r] + lo[r]; }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     m = lane & 31;
   }
+};
+
+template <> struct Mma<BF16> {
+  using Frag = bf16x8;
+  using Acc = f32x16;
   static constexpr int NACC = 16;
+  __device__ static inline Frag read(const __bf16* lds, int, int row0, int kk, int LDK, int lane) {
+    return *reinterpret_cast<const bf16x8*>(lds + (row0 + (lane & 31)) * LDK + kk + 8 * (lane >> 5));
+  }
+  __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
+  }
+  __device__ static inline void coord(int lane, int r, int& n, int& m) {
+    n = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    m = lane & 31;
+  }
 };
 
 template <> struct Mma<FP8> {
   using Frag = i32x8;
   using Acc = f32x4;
-  static constexpr int SUB = 16, KF = 128;
-  __device__ static inline Frag read(const uint8_t* lds, int, int row0, int kk, int LDK, int lane, int) {
+  static constexpr int NACC = 4;
+  __device__ static inline Frag read(const uint8_t* lds, int, int row0, int kk, int LDK, int lane) {
     const uint8_t* p = lds + (row0 + (lane & 15)) * LDK + kk + 32 * (lane >> 4);
     const i32x4 lo = *reinterpret_cast<const i32x4*>(p);
     const i32x4 hi = *reinterpret_cast<const i32x4*>(p + 16);
     Frag f = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     return f;
   }
-  __device__ static inline void zero(Acc& a, Acc&) { a = Acc{}; }
-  __device__ static inline void mma(Acc& acc, Acc&, const Frag& b, const Frag& a) {
+  __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
     // operand formats 0/0 = FP8 e4m3; block scales 127 = 2^0.
     acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(b, a, acc, 0, 0, 0, 127, 0, 127);
   }
-  __device__ static inline float value(const Acc& a, const Acc&, int r) { return a[r]; }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = 4 * (lane >> 4) + r;
     m = lane & 15;
   }
-  static constexpr int NACC = 4;
 };
-
-// Two-accumulator precisions keep a second accumulator; others keep a dummy.
-template <int P> struct SecondAcc { static constexpr bool USED = false; };
-template <> struct SecondAcc<BF16X3> { static constexpr bool USED = true; };
 
 // ---------------------------------------------------------------------------
 // Tile scheduling: spread the grid over the 8 XCDs in contiguous chunks (each
@@ -402,10 +415,11 @@ struct TileShape {
   static_assert(BK % TR::KF == 0, "k tile");
 };
 
-// One workgroup computes one BM x BN tile of C over the whole k range.
+// One BM x BN tile (tm, tn) of C over the whole k range, computed by the 256
+// threads of a workgroup.  `smem` holds two LDS stages (TileShape::LDS_BYTES).
 template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB>
-__global__ void __launch_bounds__(256)
-gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
+__device__ inline void gett_tile(const GettParams<typename PrecTraits<P>::TG>& p, int tm, int tn,
+                                 typename PrecTraits<P>::TL* smem) {
   using TR = PrecTraits<P>;
   using TG = typename TR::TG;
   using TL = typename TR::TL;
@@ -413,18 +427,10 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   using MM = Mma<P>;
   constexpr int LDK = S::LDK;
 
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  TL* smem = reinterpret_cast<TL*>(smem_raw);
-
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = t >> 6;
   const int wm = w % WM, wn = w / WM;
-
-  const int nblocks = p.tilesM * p.tilesN;
-  const int logical = xcd_chunked(blockIdx.x, nblocks);
-  int tm, tn;
-  tile_coords(logical, p.tilesM, p.tilesN, p.groupM, tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
 
   TileLoader<P, BM, BK, MODEA> la;
@@ -432,14 +438,11 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   la.init(t, p.offAm, p.ordA, m0);
   lb.init(t, p.offBn, p.ordB, n0);
 
-  typename MM::Acc acc[S::SN][S::SM], acc2[SecondAcc<P>::USED ? S::SN : 1][SecondAcc<P>::USED ? S::SM : 1];
+  typename MM::Acc acc[S::SN][S::SM];
 #pragma unroll
   for (int i = 0; i < S::SN; ++i)
 #pragma unroll
-    for (int j = 0; j < S::SM; ++j) {
-      if (SecondAcc<P>::USED) MM::zero(acc[i][j], acc2[i][j]);
-      else MM::zero(acc[i][j], acc2[0][0]);
-    }
+    for (int j = 0; j < S::SM; ++j) acc[i][j] = typename MM::Acc{};
 
   // FP8 per-tensor scales map max|X| onto the largest finite e4m3 value.
   float sA = 1.f, sB = 1.f, descale = 1.f;
@@ -450,6 +453,7 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     sB = 448.f / b;
     descale = (a / 448.f) * (b / 448.f);
   }
+
   la.fetch_k(p.offAk, 0, t);
   lb.fetch_k(p.offBk, 0, t);
   la.load(p.A, t);
@@ -479,18 +483,13 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     for (int kk = 0; kk < BK; kk += TR::KF) {
       typename MM::Frag af[S::SM], bf[S::SN];
 #pragma unroll
-      for (int j = 0; j < S::SM; ++j)
-        af[j] = MM::read(As, S::PLANE_A, wm * S::WTM + j * TR::SUB, kk, LDK, lane, 0);
+      for (int j = 0; j < S::SM; ++j) af[j] = MM::read(As, S::PLANE_A, wm * S::WTM + j * TR::SUB, kk, LDK, lane);
+#pragma unroll
+      for (int i = 0; i < S::SN; ++i) bf[i] = MM::read(Bs, S::PLANE_B, wn * S::WTN + i * TR::SUB, kk, LDK, lane);
 #pragma unroll
       for (int i = 0; i < S::SN; ++i)
-        bf[i] = MM::read(Bs, S::PLANE_B, wn * S::WTN + i * TR::SUB, kk, LDK, lane, 0);
 #pragma unroll
-      for (int i = 0; i < S::SN; ++i)
-#pragma unroll
-        for (int j = 0; j < S::SM; ++j) {
-          if (SecondAcc<P>::USED) MM::mma(acc[i][j], acc2[i][j], bf[i], af[j]);
-          else MM::mma(acc[i][j], acc2[0][0], bf[i], af[j]);
-        }
+        for (int j = 0; j < S::SM; ++j) MM::mma(acc[i][j], bf[i], af[j]);
     }
     if (more) {
       TL* Ns = smem + (stage ^ 1) * S::STAGE;
@@ -525,9 +524,7 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
 #pragma unroll
       for (int r = 0; r < MM::NACC; ++r) {
         if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
-          TG val;
-          if (SecondAcc<P>::USED) val = (TG)MM::value(acc[i][j], acc2[i][j], r);
-          else val = (TG)MM::value(acc[i][j], acc2[0][0], r);
+          TG val = (TG)acc[i][j][r];
           if (P == FP8) val = (TG)((float)val * descale);
           TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
           if (beta == TG(0)) {
@@ -539,6 +536,18 @@ gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       }
     }
   }
+}
+
+// One workgroup per C tile; the grid walks the tiles XCD-chunked and grouped.
+template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB>
+__global__ void __launch_bounds__(256)
+gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nblocks = p.tilesM * p.tilesN;
+  const int logical = xcd_chunked(blockIdx.x, nblocks);
+  int tm, tn;
+  tile_coords(logical, p.tilesM, p.tilesN, p.groupM, tm, tn);
+  gett_tile<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, tm, tn, reinterpret_cast<typename PrecTraits<P>::TL*>(smem_raw));
 }
 
 }  // namespace tccg

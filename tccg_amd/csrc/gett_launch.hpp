@@ -19,6 +19,7 @@ constexpr TileCfg kF32Tiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {12
                                  {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}};
 constexpr TileCfg kF64Tiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}, {192, 128, 16, 2, 2}};
 constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}};
+constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}};
 constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192, 128, 128, 2, 2}};
 
 inline const TileCfg* tiles_of(int prec, int* n) {
@@ -27,6 +28,7 @@ inline const TileCfg* tiles_of(int prec, int* n) {
     case tccg::F64: *n = (int)(sizeof(kF64Tiles) / sizeof(TileCfg)); return kF64Tiles;
     case tccg::BF16X3: *n = (int)(sizeof(kBfTiles) / sizeof(TileCfg)); return kBfTiles;
     case tccg::FP8: *n = (int)(sizeof(kF8Tiles) / sizeof(TileCfg)); return kF8Tiles;
+    case tccg::BF16: *n = (int)(sizeof(kBf1Tiles) / sizeof(TileCfg)); return kBf1Tiles;
     default: *n = 0; return nullptr;
   }
 }

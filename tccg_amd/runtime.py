@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 from . import build as _build
-from .plan import F32, F64, BF16X3, FP8, GettPlan, TILES
+from .plan import BF16, BF16X3, F32, F64, FP8, GettPlan, TILES
 
 
 class GettRuntimeError(RuntimeError):
@@ -37,7 +37,8 @@ class _CPlan(ctypes.Structure):
 
 
 EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_gett_f64",
-                    "tccg_gett_bf16x3", "tccg_gett_fp8", "tccg_amax_f32", "tccg_last_error"]
+                    "tccg_gett_bf16x3", "tccg_gett_fp8", "tccg_gett_bf16", "tccg_amax_f32", "tccg_last_error",
+                    "tccg_batch_num_variants", "tccg_batch_variant", "tccg_gett_batch_f32"]
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -60,11 +61,14 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.tccg_num_tiles.argtypes = [ctypes.c_int]
         for name, T in (("tccg_gett_f32", f32), ("tccg_gett_bf16x3", f32), ("tccg_gett_fp8", f32),
-                        ("tccg_gett_f64", f64)):
+                        ("tccg_gett_bf16", f32), ("tccg_gett_f64", f64)):
             fn = getattr(lib, name)
             fn.argtypes = [vp, vp, vp, T, T, ctypes.POINTER(_CPlan), vp]
             fn.restype = ctypes.c_int
         lib.tccg_amax_f32.argtypes = [vp, ctypes.c_int64, vp, vp]
+        lib.tccg_gett_batch_f32.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
+        lib.tccg_gett_batch_f32.restype = ctypes.c_int
+        lib.tccg_batch_variant.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 5
         lib.tccg_last_error.restype = ctypes.c_char_p
         # The Python tile table must agree with the library's.
         for prec, tiles in TILES.items():
@@ -154,7 +158,7 @@ class DevicePlan:
             if rc != 0:
                 raise GettRuntimeError(last_error())
         fn = {F32: lib.tccg_gett_f32, F64: lib.tccg_gett_f64, BF16X3: lib.tccg_gett_bf16x3,
-              FP8: lib.tccg_gett_fp8}[prec]
+              FP8: lib.tccg_gett_fp8, BF16: lib.tccg_gett_bf16}[prec]
         rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
         if rc != 0:
             raise GettRuntimeError("GETT launch failed: %s" % last_error())
