@@ -34,7 +34,8 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--max-implementations", type=int, default=8,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
-    ap.add_argument("--only", default=None, help="run one form only: f32, bw, f64, bf16x3, fp8 (profiling)")
+    ap.add_argument("--only", default=None,
+                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -49,18 +50,21 @@ def rel_err(x, ref) -> float:
     return float(d / ref.norm())
 
 
-def fp64_reference(c, A, B):
-    """alpha*A.B in float64 on the GPU with torch.einsum over contiguous
-    operands (an independent library path, not this backend)."""
+def _einsum_eq(c) -> str:
     import string
-    import torch
     letters = {}
     pool = iter(string.ascii_letters)
     for lab in [i.label for i in c.C0.indices + c.A0.indices + c.B0.indices]:
         letters.setdefault(lab, next(pool))
     rev = lambda T: "".join(letters[i.label] for i in reversed(T.indices))
-    eq = "%s,%s->%s" % (rev(c.A0), rev(c.B0), rev(c.C0))
-    return torch.einsum(eq, A.double().contiguous(), B.double().contiguous()) * c.alpha
+    return "%s,%s->%s" % (rev(c.A0), rev(c.B0), rev(c.C0))
+
+
+def fp64_reference(c, A, B):
+    """alpha*A.B in float64 on the GPU with torch.einsum over contiguous
+    operands (an independent library path, not this backend)."""
+    import torch
+    return torch.einsum(_einsum_eq(c), A.double().contiguous(), B.double().contiguous()) * c.alpha
 
 
 def timed_steps(dp, A, B, C, steps: int, warmup: int):
@@ -138,6 +142,9 @@ def run_form(name, wl, prec, args, log):
         "kernel_gbs": bytes_ / kern / 1e9, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
     }
     if prec != "f64" and wl in ("abcd-aebf-dfce", "abcdef-dgfa-gceb"):
+        C.zero_()
+        dp.run(A, B, C)
+        torch.cuda.synchronize()
         ref = fp64_reference(c, A, B)
         res["rel_err_vs_fp64"] = rel_err(C, ref)
         del ref
@@ -150,6 +157,96 @@ def run_form(name, wl, prec, args, log):
     del A, B, C
     torch.cuda.empty_cache()
     return res
+
+
+def run_batch(args, log):
+    """512 FP32 contractions from the 4 batch patterns, every extent uniform over
+    multiples of 8 in [8, 96] (seed 0); tiles tuned per pattern; one
+    HIP-graph-captured persistent launch per step."""
+    import torch
+    from tccg_amd.batch import GettBatch, autotune_patterns, random_batch
+    cs = random_batch(512, seed=0)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    As, Bs, Cs = [], [], []
+    for c in cs:
+        sa, sb, sc = c.shapes()
+        As.append(torch.rand(sa, device="cuda", generator=g) * 2 - 1)
+        Bs.append(torch.rand(sb, device="cuda", generator=g) * 2 - 1)
+        Cs.append(torch.zeros(sc, device="cuda"))
+    t0 = time.perf_counter()
+    shapes = autotune_patterns(cs, As, Bs, Cs, reps=2, verbose=args.verbose)
+    tune_s = time.perf_counter() - t0
+    b = GettBatch(cs, shapes).bind(As, Bs, Cs)
+    graph = b.capture()
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(s)
+        graph.replay()
+        e1.record(s)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    kern = sum(e0.elapsed_time(e1) for e0, e1 in evs) / args.steps * 1e-3
+    errs = []
+    for i in range(0, len(cs), 32):   # every 32nd problem against FP64 einsum on the GPU
+        ref = fp64_reference(cs[i], As[i], Bs[i])
+        errs.append(rel_err(Cs[i], ref))
+    flops = b.flops()
+    res = {"workload": "512 FP32 contractions, 4 patterns, extents U{8..96 step 8}, seed 0",
+           "problems": len(cs), "tiles": b._bound["nwork"], "grid": b._bound["grid"],
+           "pattern_shapes": {k: int(v) for k, v in shapes.items()}, "tune_seconds": round(tune_s, 2),
+           "wall_ms_per_step": wall * 1e3, "kernel_ms": kern * 1e3, "gflops": flops / wall / 1e9,
+           "kernel_gflops": flops / kern / 1e9, "algorithmic_flops": flops,
+           "max_rel_err_vs_fp64_sampled": max(errs)}
+    log("batch   512 problems        %9.1f GFLOP/s (graph replay %.3f ms, %d tiles)" % (
+        res["kernel_gflops"], res["kernel_ms"], res["tiles"]))
+    del As, Bs, Cs, b, graph
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_grad(args, log):
+    """Forward + backward (two adjoint GETTs) of C[a,b,c,d] = A[a,k,l,b] * B[l,c,k,d],
+    72/32, loss 0.5*||C - T||^2, FP32; gradients against FP64 torch.einsum autograd."""
+    import torch
+    from tccg_amd import Contraction
+    from tccg_amd.autograd import adjoints, contract
+    from tccg_amd.workloads import get
+    c = Contraction.from_tccg(get("abcd-aklb-lckd"))
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = torch.randn(c.A0.torch_shape(), device="cuda", generator=g)
+    B = torch.randn(c.B0.torch_shape(), device="cuda", generator=g)
+    T = torch.randn(c.C0.torch_shape(), device="cuda", generator=g)
+    out = {}
+    for prec in ("f32", "bf16x3", "bf16"):
+        A1, B1 = A.clone().requires_grad_(), B.clone().requires_grad_()
+
+        def step():
+            A1.grad = None
+            B1.grad = None
+            L = 0.5 * ((contract(A1, B1, c, prec) - T) ** 2).sum()
+            L.backward()
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        A2, B2 = A.double().requires_grad_(), B.double().requires_grad_()
+        eq = fp64_reference.__globals__["_einsum_eq"](c)
+        L2 = 0.5 * ((torch.einsum(eq, A2, B2) - T.double()) ** 2).sum()
+        L2.backward()
+        out[prec] = {"step_ms": wall * 1e3, "gett_gflops_per_step": 3 * c.flops() / wall / 1e9,
+                     "rel_err_dA": rel_err(A1.grad, A2.grad), "rel_err_dB": rel_err(B1.grad, B2.grad)}
+        log("grad    %-7s abcd-aklb-lckd %8.1f GFLOP/s (3 GETTs + elementwise, %.3f ms/step) dA err %.2e" % (
+            prec, out[prec]["gett_gflops_per_step"], wall * 1e3, out[prec]["rel_err_dA"]))
+    return out
 
 
 def cpu_baseline(args, log):
@@ -188,13 +285,23 @@ def main():
         "bw": ("abc-akb-kc", "f32"),
         "f64": ("abcd-aebf-dfce", "f64"),
         "bf16x3": ("abcd-aebf-dfce", "bf16x3"),
+        "bf16": ("abcd-aebf-dfce", "bf16"),
+        "fp8_f32": ("abcdef-dgfa-gceb", "f32"),
+        "fp8_bf16x3": ("abcdef-dgfa-gceb", "bf16x3"),
         "fp8": ("abcdef-dgfa-gceb", "fp8"),
+        "batch": None,
+        "grad": None,
     }
     if args.only:
         forms = {args.only: forms[args.only]}
     results = {}
-    for key, (wl, prec) in forms.items():
-        results[key] = run_form(key, wl, prec, args, log)
+    for key, spec in forms.items():
+        if key == "batch":
+            results[key] = run_batch(args, log)
+        elif key == "grad":
+            results[key] = run_grad(args, log)
+        else:
+            results[key] = run_form(key, spec[0], spec[1], args, log)
     head = results.get("f32") or next(iter(results.values()))
     details = {}
     if "bf16x3" in results and "f32" in results:
@@ -203,6 +310,9 @@ def main():
                                                    / max(results["f32"]["rel_err_vs_fp64"], 1e-30))
     if "bw" in results:
         details["bw_hbm_roofline_frac"] = results["bw"]["kernel_gbs"] / PEAK_HBM_GBS
+    for k in ("fp8_f32", "fp8_bf16x3", "fp8"):
+        if k in results:
+            details["%s_rel_err_vs_fp64" % k] = results[k].get("rel_err_vs_fp64")
     cpu = None if (args.no_cpu_baseline or args.only) else cpu_baseline(args, log)
     achieved_tf = head["kernel_gflops"] / 1e3
     line = {

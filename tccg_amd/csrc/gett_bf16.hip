@@ -14,6 +14,7 @@ int dispatch_tile<tccg::BF16>(int tile, int ma, int mb, const tccg::GettParams<f
     TCCG_TILE_CASE(kBf1Tiles, 0)
     TCCG_TILE_CASE(kBf1Tiles, 1)
     TCCG_TILE_CASE(kBf1Tiles, 2)
+    TCCG_TILE_CASE(kBf1Tiles, 3)
     default: return fail(1, "invalid tile id");
   }
 }

@@ -14,6 +14,7 @@ int dispatch_tile<tccg::BF16X3>(int tile, int ma, int mb, const tccg::GettParams
     TCCG_TILE_CASE(kBfTiles, 0)
     TCCG_TILE_CASE(kBfTiles, 1)
     TCCG_TILE_CASE(kBfTiles, 2)
+    TCCG_TILE_CASE(kBfTiles, 3)
     default: return fail(1, "invalid tile id");
   }
 }

@@ -16,6 +16,7 @@ int dispatch_tile<tccg::F32>(int tile, int ma, int mb, const tccg::GettParams<fl
     TCCG_TILE_CASE(kF32Tiles, 2)
     TCCG_TILE_CASE(kF32Tiles, 3)
     TCCG_TILE_CASE(kF32Tiles, 4)
+    TCCG_TILE_CASE(kF32Tiles, 5)
     default: return fail(1, "invalid tile id");
   }
 }

@@ -16,10 +16,10 @@ struct TileCfg { int BM, BN, BK, WM, WN; };
 
 // Tile configurations per precision; ids index these arrays.
 constexpr TileCfg kF32Tiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 256, 16, 2, 2},
-                                 {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}};
+                                 {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kF64Tiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}, {192, 128, 16, 2, 2}};
-constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}};
-constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}};
+constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
+constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192, 128, 128, 2, 2}};
 
 inline const TileCfg* tiles_of(int prec, int* n) {

@@ -38,11 +38,11 @@ FREE4, FREE1, KDIM4, KDIM1 = 0, 1, 2, 3
 # Tile shapes per precision: (BM, BN, BK, LDS bytes per workgroup).  Must match
 # the tables of csrc/gett_launch.hpp (checked against the library at load time).
 TILES: Dict[int, List[Tuple[int, int, int]]] = {
-    F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16)],
+    F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16), (128, 128, 16)],
     F64: [(128, 128, 16), (64, 64, 16), (192, 128, 16)],
-    BF16X3: [(128, 128, 32), (64, 64, 32), (192, 128, 32)],
+    BF16X3: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
     FP8: [(128, 128, 128), (64, 64, 128), (192, 128, 128)],
-    BF16: [(128, 128, 32), (64, 64, 32), (192, 128, 32)],
+    BF16: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
 }
 _LDS_PAD = {F32: (4, 4), F64: (2, 8), BF16X3: (8, 2), FP8: (16, 1), BF16: (8, 2)}  # (pad elements, element bytes)
 _PLANES = {F32: 1, F64: 1, BF16X3: 3, FP8: 1, BF16: 1}

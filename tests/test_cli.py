@@ -95,3 +95,23 @@ def test_emit_fp8_reduces_amax():
     c = Contraction.from_tccg(WORKLOADS["example"])
     src = codegen.emit_source(c.plan(FP8), WORKLOADS["example"])
     assert "amax_reduce" in src and "t->amax" in src
+
+
+def test_generated_tables_are_exported_device_symbols(tmp_path):
+    """The embedded offset tables must be exported symbols of the device code
+    object (a table dropped by the device compile, or kept as a local symbol,
+    leaves the launch without valid table addresses)."""
+    import re
+    from tccg_amd import build
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    name = "gett_regress"
+    src = tmp_path / (name + ".hip")
+    src.write_text(codegen.emit_source(c.plan(F32), WORKLOADS["example"], name))
+    asm = tmp_path / "dev.s"
+    cmd = [build.HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-I", str(build.CSRC), "--cuda-device-only",
+           "-S", str(src), "-o", str(asm)]
+    subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    text = asm.read_text()
+    for key in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB"):
+        sym = "tccg_%s_%s" % (name, key)
+        assert re.search(r"^\s*\.globl\s+%s\s*$" % sym, text, flags=re.M), sym
