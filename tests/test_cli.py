@@ -88,7 +88,7 @@ def test_emit_source_tables_and_swap():
     p = c.plan(F32)
     src = codegen.emit_source(p, "C[b,a] = A[a,k] * B[k,b]")
     assert "const float* opA = 1 ? B : A;" in src
-    assert "kOffAm[%d]" % p.Mp in src and "kOffCn[%d]" % p.Np in src
+    assert "tccg_gett_offAm[%d]" % p.Mp in src and "tccg_gett_offCn[%d]" % p.Np in src
 
 
 def test_emit_fp8_reduces_amax():
