@@ -204,8 +204,12 @@ struct TileLoader {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       const int b = t + 256 * p;
+      // FREE4: the k-chunk runs fastest across lanes, so one LDS store cycle
+      // covers whole packed rows (no bank conflicts) while each k-row of the
+      // gather still reads full 128-byte segments along the free index.
+      // FREE1: lanes along the free index.  KDIM: lanes along k.
       int fg, kg;
-      if (FREE_LANES) { fg = b % (F / VF); kg = b / (F / VF); }
+      if (MODE == FREE1) { fg = b % (F / VF); kg = b / (F / VF); }
       else { kg = b % (BK / 4); fg = b / (BK / 4); }
       kl[p] = kg * 4;
       const bool ok = active(t, p);
@@ -234,6 +238,7 @@ struct TileLoader {
       if (!active(t, p)) continue;
       const bool fok = foff[p] != kInvalid;
       unsigned m = 0;
+      if (!fok) m = 0u;
       if (MODE == FREE4) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -265,12 +270,15 @@ struct TileLoader {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       if (!active(t, p)) continue;
+      // Invalid positions only occur in padded k-steps and ragged tiles.
+      if (okm[p] != 0xfu) {
 #pragma unroll
-      for (int e = 0; e < VF; ++e) {
+        for (int e = 0; e < VF; ++e)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[p][e][j] = ((okm[p] >> j) & 1u) ? v[p][e][j] : TG(0);
-        RowWriter<P>::write(lds, plane_stride, row[p][e] * LDK + kl[p], v[p][e], scale);
+          for (int j = 0; j < 4; ++j) v[p][e][j] = ((okm[p] >> j) & 1u) ? v[p][e][j] : TG(0);
       }
+#pragma unroll
+      for (int e = 0; e < VF; ++e) RowWriter<P>::write(lds, plane_stride, row[p][e] * LDK + kl[p], v[p][e], scale);
     }
   }
 };
