@@ -32,7 +32,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--max-implementations", type=int, default=8,
+    ap.add_argument("--max-implementations", type=int, default=12,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
     ap.add_argument("--only", default=None,
                     help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad (profiling)")

@@ -79,6 +79,9 @@ typedef struct tccg_gett_plan {
   int32_t modeA;       /* enum tccg_load_mode of A                           */
   int32_t modeB;       /* enum tccg_load_mode of B                           */
   int32_t groupM;      /* m-tiles per scheduling group (>= 1)                */
+  int32_t grid;        /* 0: one workgroup per C tile; > 0: that many       */
+                       /* persistent workgroups walk the tiles, prefetching */
+                       /* the next tile's first k-step during the epilogue  */
   const float* amax;   /* TCCG_FP8 only: device float[2] = max|A|, max|B|   */
 } tccg_gett_plan;
 

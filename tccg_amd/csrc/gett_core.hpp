@@ -200,7 +200,8 @@ struct TileLoader {
 
   // Loader slots are walked lane-major; ord maps a slot to the tile row it
   // fills, so the lanes of a wave can follow the operand's stride-one index.
-  __device__ inline void init(int t, const uint32_t* __restrict__ offF, const int* __restrict__ ord, int f0) {
+  // Rows and k-chunks are the same for every tile.
+  __device__ inline void init_rows(int t, const int* __restrict__ ord) {
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
       const int b = t + 256 * p;
@@ -215,8 +216,18 @@ struct TileLoader {
       const bool ok = active(t, p);
 #pragma unroll
       for (int e = 0; e < VF; ++e) row[p][e] = ok ? ord[fg * VF + e] : 0;
-      foff[p] = ok ? offF[f0 + row[p][0]] : kInvalid;
     }
+  }
+
+  // Free-index offsets of this thread's micro-blocks in the tile starting at f0.
+  __device__ inline void tile_offsets(const uint32_t* __restrict__ offF, int f0, int t, uint32_t* out) const {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) out[p] = active(t, p) ? offF[f0 + row[p][0]] : kInvalid;
+  }
+
+  __device__ inline void init(int t, const uint32_t* __restrict__ offF, const int* __restrict__ ord, int f0) {
+    init_rows(t, ord);
+    tile_offsets(offF, f0, t, foff);
   }
 
   // Table entries of the k-step starting at k0 (issued one step ahead so that
@@ -546,16 +557,197 @@ __device__ inline void gett_tile(const GettParams<typename PrecTraits<P>::TG>& p
   }
 }
 
-// One workgroup per C tile; the grid walks the tiles XCD-chunked and grouped.
+// A workgroup walks the C tiles blockIdx.x, blockIdx.x + gridDim.x, ... (in
+// the XCD-chunked, grouped order).  The first gathers of its next tile are
+// issued before the epilogue of the current one, so their latency overlaps
+// the C stores; with gridDim.x == number of tiles this is one tile per
+// workgroup.
 template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB>
+__device__ inline void gett_walk(const GettParams<typename PrecTraits<P>::TG>& p, typename PrecTraits<P>::TL* smem) {
+  using TR = PrecTraits<P>;
+  using TG = typename TR::TG;
+  using TL = typename TR::TL;
+  using S = TileShape<P, BM, BN, BK, WM, WN>;
+  using MM = Mma<P>;
+  constexpr int LDK = S::LDK;
+
+  const int nblocks = p.tilesM * p.tilesN;
+  int v = blockIdx.x;
+  if (v >= nblocks) return;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const int wm = w % WM, wn = w / WM;
+  const int KT = p.kTiles;
+
+  float sA = 1.f, sB = 1.f, descale = 1.f;
+  if (P == FP8) {
+    const float a = p.amax[0] > 0.f ? p.amax[0] : 1.f;
+    const float b = p.amax[1] > 0.f ? p.amax[1] : 1.f;
+    sA = 448.f / a;
+    sB = 448.f / b;
+    descale = (a / 448.f) * (b / 448.f);
+  }
+
+  TileLoader<P, BM, BK, MODEA> la;
+  TileLoader<P, BN, BK, MODEB> lb;
+  la.init_rows(t, p.ordA);
+  lb.init_rows(t, p.ordB);
+  int tm, tn;
+  tile_coords(xcd_chunked(v, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
+  la.tile_offsets(p.offAm, tm * BM, t, la.foff);
+  lb.tile_offsets(p.offBn, tn * BN, t, lb.foff);
+  la.fetch_k(p.offAk, 0, t);
+  lb.fetch_k(p.offBk, 0, t);
+  // k-step 0 table entries are the same for every tile
+  uint32_t kn0A[decltype(la)::PT][decltype(la)::NJ], kn0B[decltype(lb)::PT][decltype(lb)::NJ];
+#pragma unroll
+  for (int q = 0; q < decltype(la)::PT; ++q)
+#pragma unroll
+    for (int j = 0; j < decltype(la)::NJ; ++j) kn0A[q][j] = la.kn[q][j];
+#pragma unroll
+  for (int q = 0; q < decltype(lb)::PT; ++q)
+#pragma unroll
+    for (int j = 0; j < decltype(lb)::NJ; ++j) kn0B[q][j] = lb.kn[q][j];
+  la.load(p.A, t);
+  lb.load(p.B, t);
+
+  while (true) {
+    const int m0 = tm * BM, n0 = tn * BN;
+    // next tile's free offsets, loaded early
+    const int vn = v + (int)gridDim.x;
+    const bool has_next = vn < nblocks;
+    int tm2 = 0, tn2 = 0;
+    uint32_t nfA[decltype(la)::PT], nfB[decltype(lb)::PT];
+    if (has_next) {
+      tile_coords(xcd_chunked(vn, nblocks), p.tilesM, p.tilesN, p.groupM, tm2, tn2);
+      la.tile_offsets(p.offAm, tm2 * BM, t, nfA);
+      lb.tile_offsets(p.offBn, tn2 * BN, t, nfB);
+    }
+    if (KT > 1) {
+      la.fetch_k(p.offAk, BK, t);
+      lb.fetch_k(p.offBk, BK, t);
+    }
+
+    typename MM::Acc acc[S::SN][S::SM];
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+      for (int j = 0; j < S::SM; ++j) acc[i][j] = typename MM::Acc{};
+
+    la.store(smem, S::PLANE_A, sA, t);
+    lb.store(smem + TR::PLANES * S::PLANE_A, S::PLANE_B, sB, t);
+    __syncthreads();
+
+    for (int kt = 0; kt < KT; ++kt) {
+      const int stage = kt & 1;
+      const bool more = (kt + 1) < KT;
+      if (more) {
+        la.load(p.A, t);
+        lb.load(p.B, t);
+        if (kt + 2 < KT) {
+          la.fetch_k(p.offAk, (kt + 2) * BK, t);
+          lb.fetch_k(p.offBk, (kt + 2) * BK, t);
+        }
+      }
+      const TL* As = smem + stage * S::STAGE;
+      const TL* Bs = As + TR::PLANES * S::PLANE_A;
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += TR::KF) {
+        typename MM::Frag af[S::SM], bf[S::SN];
+#pragma unroll
+        for (int j = 0; j < S::SM; ++j) af[j] = MM::read(As, S::PLANE_A, wm * S::WTM + j * TR::SUB, kk, LDK, lane);
+#pragma unroll
+        for (int i = 0; i < S::SN; ++i) bf[i] = MM::read(Bs, S::PLANE_B, wn * S::WTN + i * TR::SUB, kk, LDK, lane);
+#pragma unroll
+        for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+          for (int j = 0; j < S::SM; ++j) MM::mma(acc[i][j], bf[i], af[j]);
+      }
+      if (more) {
+        TL* Ns = smem + (stage ^ 1) * S::STAGE;
+        la.store(Ns, S::PLANE_A, sA, t);
+        lb.store(Ns + TR::PLANES * S::PLANE_A, S::PLANE_B, sB, t);
+      }
+      __syncthreads();
+    }
+
+    // C offsets of this tile, then the next tile's first gathers, then the stores
+    uint32_t cm[S::SM], cn[S::SN][MM::NACC];
+#pragma unroll
+    for (int j = 0; j < S::SM; ++j) {
+      int n_l, m_l;
+      MM::coord(lane, 0, n_l, m_l);
+      cm[j] = p.offCm[m0 + wm * S::WTM + j * TR::SUB + m_l];
+    }
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+      for (int r = 0; r < MM::NACC; ++r) {
+        int n_l, m_l;
+        MM::coord(lane, r, n_l, m_l);
+        cn[i][r] = p.offCn[n0 + wn * S::WTN + i * TR::SUB + n_l];
+      }
+    if (has_next) {
+#pragma unroll
+      for (int q = 0; q < decltype(la)::PT; ++q) {
+        la.foff[q] = nfA[q];
+#pragma unroll
+        for (int j = 0; j < decltype(la)::NJ; ++j) la.kn[q][j] = kn0A[q][j];
+      }
+#pragma unroll
+      for (int q = 0; q < decltype(lb)::PT; ++q) {
+        lb.foff[q] = nfB[q];
+#pragma unroll
+        for (int j = 0; j < decltype(lb)::NJ; ++j) lb.kn[q][j] = kn0B[q][j];
+      }
+      la.load(p.A, t);
+      lb.load(p.B, t);
+    }
+
+    const TG alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int j = 0; j < S::SM; ++j) {
+#pragma unroll
+      for (int i = 0; i < S::SN; ++i) {
+#pragma unroll
+        for (int r = 0; r < MM::NACC; ++r) {
+          if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
+            TG val = (TG)acc[i][j][r];
+            if (P == FP8) val = (TG)((float)val * descale);
+            TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
+            if (beta == TG(0)) {
+              __builtin_nontemporal_store(alpha * val, dst);
+            } else {
+              *dst = alpha * val + beta * (*dst);
+            }
+          }
+        }
+      }
+    }
+    if (!has_next) break;
+    v = vn;
+    tm = tm2;
+    tn = tn2;
+  }
+}
+
+// PERSIST == false: one tile per workgroup (gridDim.x == tiles).
+// PERSIST == true: each workgroup walks several tiles with cross-tile
+// prefetch (gridDim.x <= tiles); worth its extra registers for short k ranges.
+template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB, bool PERSIST>
 __global__ void __launch_bounds__(256)
 gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nblocks = p.tilesM * p.tilesN;
-  const int logical = xcd_chunked(blockIdx.x, nblocks);
-  int tm, tn;
-  tile_coords(logical, p.tilesM, p.tilesN, p.groupM, tm, tn);
-  gett_tile<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, tm, tn, reinterpret_cast<typename PrecTraits<P>::TL*>(smem_raw));
+  auto* smem = reinterpret_cast<typename PrecTraits<P>::TL*>(smem_raw);
+  if constexpr (PERSIST) {
+    gett_walk<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, smem);
+  } else {
+    const int nblocks = p.tilesM * p.tilesN;
+    int tm, tn;
+    tile_coords(xcd_chunked(blockIdx.x, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
+    gett_tile<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, tm, tn, smem);
+  }
 }
 
 }  // namespace tccg

@@ -16,7 +16,8 @@ struct TileCfg { int BM, BN, BK, WM, WN; };
 
 // Tile configurations per precision; ids index these arrays.
 constexpr TileCfg kF32Tiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 256, 16, 2, 2},
-                                 {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}, {128, 128, 16, 2, 2}};
+                                 {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
+                                 {128, 128, 8, 2, 2},  {64, 384, 8, 1, 4},   {128, 256, 8, 2, 2}};
 constexpr TileCfg kF64Tiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}, {192, 128, 16, 2, 2}};
 constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
@@ -33,10 +34,10 @@ inline const TileCfg* tiles_of(int prec, int* n) {
   }
 }
 
-template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB>
-int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, hipStream_t stream) {
+template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB, bool PERSIST>
+int launch_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t stream) {
   using S = tccg::TileShape<P, BM, BN, BK, WM, WN>;
-  auto kern = tccg::gett_kernel<P, BM, BN, BK, WM, WN, MA, MB>;
+  auto kern = tccg::gett_kernel<P, BM, BN, BK, WM, WN, MA, MB, PERSIST>;
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -44,21 +45,33 @@ int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, hi
     if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     attr_set = true;
   }
-  const int nblocks = prm.tilesM * prm.tilesN;
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(256), S::LDS_BYTES, stream, prm);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), S::LDS_BYTES, stream, prm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
   return 0;
 }
 
+// grid == 0: one tile per workgroup; grid > 0: persistent walk over that many workgroups.
+template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB>
+int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t stream) {
+  const int nblocks = prm.tilesM * prm.tilesN;
+  // The persistent walk is built for the vector load modes only; plans with a
+  // scalar mode run one tile per workgroup.
+  constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4) && (MB == tccg::FREE4 || MB == tccg::KDIM4);
+  if constexpr (kVec) {
+    if (grid > 0 && grid < nblocks) return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, true>(prm, grid, stream);
+  }
+  return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, false>(prm, nblocks, stream);
+}
+
 template <int P, int BM, int BN, int BK, int WM, int WN>
-int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, hipStream_t s) {
+int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t s) {
 #define TCCG_MB(MA_)                                                                       \
   switch (mb) {                                                                            \
-    case 0: return launch_one<P, BM, BN, BK, WM, WN, MA_, 0>(prm, s);                      \
-    case 1: return launch_one<P, BM, BN, BK, WM, WN, MA_, 1>(prm, s);                      \
-    case 2: return launch_one<P, BM, BN, BK, WM, WN, MA_, 2>(prm, s);                      \
-    case 3: return launch_one<P, BM, BN, BK, WM, WN, MA_, 3>(prm, s);                      \
+    case 0: return launch_one<P, BM, BN, BK, WM, WN, MA_, 0>(prm, grid, s);                \
+    case 1: return launch_one<P, BM, BN, BK, WM, WN, MA_, 1>(prm, grid, s);                \
+    case 2: return launch_one<P, BM, BN, BK, WM, WN, MA_, 2>(prm, grid, s);                \
+    case 3: return launch_one<P, BM, BN, BK, WM, WN, MA_, 3>(prm, grid, s);                \
     default: return fail(1, "invalid load mode of B");                                     \
   }
   switch (ma) {
@@ -74,7 +87,7 @@ int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTrait
 
 template <int P>
 int dispatch_tile(int tile, int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm,
-                  hipStream_t s);
+                  int grid, hipStream_t s);
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
@@ -114,7 +127,8 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
-  return dispatch_tile<P>(plan->tile, plan->modeA, plan->modeB, prm, (hipStream_t)stream);
+  if (plan->grid < 0) return fail(1, "grid must be >= 0");
+  return dispatch_tile<P>(plan->tile, plan->modeA, plan->modeB, prm, plan->grid, (hipStream_t)stream);
 }
 
 

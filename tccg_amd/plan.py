@@ -38,12 +38,16 @@ FREE4, FREE1, KDIM4, KDIM1 = 0, 1, 2, 3
 # Tile shapes per precision: (BM, BN, BK, LDS bytes per workgroup).  Must match
 # the tables of csrc/gett_launch.hpp (checked against the library at load time).
 TILES: Dict[int, List[Tuple[int, int, int]]] = {
-    F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16), (128, 128, 16)],
+    F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16), (128, 128, 16),
+          (128, 128, 8), (64, 384, 8), (128, 256, 8)],
     F64: [(128, 128, 16), (64, 64, 16), (192, 128, 16)],
     BF16X3: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
     FP8: [(128, 128, 128), (64, 64, 128), (192, 128, 128)],
     BF16: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
 }
+# Wave arrangement (WM x WN) of each tile; (2, 2) unless listed.
+WAVES: Dict[Tuple[int, int], Tuple[int, int]] = {(F32, 7): (1, 4)}
+
 _LDS_PAD = {F32: (4, 4), F64: (2, 8), BF16X3: (8, 2), FP8: (16, 1), BF16: (8, 2)}  # (pad elements, element bytes)
 _PLANES = {F32: 1, F64: 1, BF16X3: 3, FP8: 1, BF16: 1}
 
@@ -270,6 +274,7 @@ class GettPlan:
     modeA: int
     modeB: int
     groupM: int = 8
+    grid: int = 0          # 0: one workgroup per tile; > 0: persistent walk
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -292,8 +297,9 @@ class GettPlan:
 
     def describe(self) -> str:
         mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
-        return ("prec=%s tile=%d (%dx%dx%d) m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
-                % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, self.lin_m.inner,
+        return ("prec=%s tile=%d (%dx%dx%d)%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
+                % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK,
+                   (" persistent grid=%d" % self.grid) if self.grid else "", self.lin_m.inner,
                    self.lin_n.inner, self.lin_k.inner, mode[self.modeA], mode[self.modeB], self.ntiles(),
                    self.Mp, self.Np, self.Kp))
 
@@ -332,7 +338,7 @@ def _ext(pb: GettProblem, labels: Sequence[str]) -> List[int]:
 
 
 def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] = None,
-               groupM: int = 8) -> GettPlan:
+               groupM: int = 8, persistent: bool = False) -> GettPlan:
     BM, BN, BK = TILES[prec][tile]
     m_l, n_l, k_l = _orders(pb)
     m_e, n_e, k_e = _ext(pb, m_l), _ext(pb, n_l), _ext(pb, k_l)
@@ -368,8 +374,14 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
 
     modeA = mode(pb.opA, m_l, offAm, ordA, BM, offAk)
     modeB = mode(pb.opB, n_l, offBn, ordB, BN, offBk)
-    return GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
+    plan = GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
                     ordA, ordB, modeA, modeB, groupM)
+    if persistent and plan.modeA in (FREE4, KDIM4) and plan.modeB in (FREE4, KDIM4):
+        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // lds_bytes(prec, BM, BN, BK)))
+        plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
+        if plan.grid >= plan.ntiles():
+            plan.grid = 0
+    return plan
 
 
 # ---------------------------------------------------------------------------
@@ -386,27 +398,33 @@ MI355X = {
 
 
 def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
-    """Roofline estimate of one launch: max(MFMA time of the padded work with
-    wave quantisation over the CUs, HBM time of A, B and C), with a penalty for
-    small tiles (more L2 traffic per flop) and scalar (non-vector) loads."""
+    """Estimate of one launch, the gfx950 analogue of Gett.estimateGFLOPS
+    (tccg/gett.py:590-687): per tile, MFMA time of the padded work plus a fixed
+    cost (gather latency and C write-back at the slot's share of HBM
+    bandwidth; mostly hidden by the persistent walk), times the number of
+    tile rounds over the CUs; bounded below by the HBM time of A, B and C."""
     pb = plan.problem
     hw = MI355X
     lds = lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK)
     per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
     slots = hw["cus"] * per_cu
     ntiles = plan.ntiles()
-    waves = math.ceil(ntiles / slots)
-    per_tile = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots)
     eff = 0.75 if plan.BM * plan.BN >= 128 * 128 else 0.55
     if plan.modeA in (FREE1, KDIM1):
-        eff *= 0.9
+        eff *= 0.85
     if plan.modeB in (FREE1, KDIM1):
-        eff *= 0.9
-    t_mfma = waves * per_tile / eff
+        eff *= 0.85
+    t_mfma = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots) / eff
     esz = 8 if plan.prec == F64 else 4
+    t_store = esz * plan.BM * plan.BN / (0.7 * hw["hbm_bytes_per_s"] / slots)
+    t_fixed = 2.0e-6 + t_store
+    if plan.grid:
+        t_fixed = 0.4e-6 + t_store * 0.6
+    rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
+    t = rounds * (t_mfma + t_fixed)
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
     t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
-    return max(t_mfma, t_hbm)
+    return max(t, t_hbm)
 
 
 def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPlan]:
@@ -418,6 +436,11 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
     for tile in range(len(TILES[prec])):
         for blocked in ((False, True) if conflict else (False,)):
             plans.append(build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM))
+            # short k ranges: also the persistent walk with cross-tile prefetch
+            if pb.K <= 256:
+                pp = build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM, persistent=True)
+                if pp.grid:
+                    plans.append(pp)
     plans.sort(key=lambda p: estimate_seconds(p))
     return plans
 

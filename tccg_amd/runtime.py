@@ -31,7 +31,7 @@ class _CPlan(ctypes.Structure):
         ("ordA", ctypes.c_void_p), ("ordB", ctypes.c_void_p),
         ("Mp", ctypes.c_int32), ("Np", ctypes.c_int32), ("Kp", ctypes.c_int32),
         ("tile", ctypes.c_int32), ("modeA", ctypes.c_int32), ("modeB", ctypes.c_int32),
-        ("groupM", ctypes.c_int32),
+        ("groupM", ctypes.c_int32), ("grid", ctypes.c_int32),
         ("amax", ctypes.c_void_p),
     ]
 
@@ -121,6 +121,7 @@ class DevicePlan:
         (c.offAm, c.offAk, c.offBn, c.offBk, c.offCm, c.offCn, c.ordA, c.ordB) = [t.data_ptr() for t in self.tables]
         c.Mp, c.Np, c.Kp = plan.Mp, plan.Np, plan.Kp
         c.tile, c.modeA, c.modeB, c.groupM = plan.tile, plan.modeA, plan.modeB, plan.groupM
+        c.grid = plan.grid
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
 
