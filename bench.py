@@ -23,7 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "GFLOP/s + roofline %; split-BF16 vs plain FP32 GETT at equal error vs FP64"
-PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "bf16x3": 2516.6 / 6.0, "fp8": 5033.0}
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6, "bf16x3": 2516.6 / 6.0, "bf16": 2516.6, "fp8": 5033.0}
 PEAK_HBM_GBS = 8000.0
 
 

@@ -45,12 +45,15 @@ _lib_lock = threading.Lock()
 
 
 def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
-    """Load (once) the in-tree GETT library; builds it only if it is absent."""
+    """Load (once) the in-tree GETT library (or the one named by $TCCG_GFX950_LIB);
+    builds the in-tree library only if it is absent."""
     global _lib
     with _lib_lock:
         if _lib is not None:
             return _lib
-        p = Path(path) if path else _build.library_path()
+        import os
+        env = os.environ.get("TCCG_GFX950_LIB")
+        p = Path(path) if path else (Path(env) if env else _build.library_path())
         if not p.exists():
             if not _build.hipcc_available():
                 raise GettRuntimeError("GETT library %s is missing and hipcc is unavailable" % p)
