@@ -30,6 +30,7 @@
 #ifndef TCCG_GFX950_H
 #define TCCG_GFX950_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -102,6 +103,39 @@ int tccg_gett_fp8(const float* A, const float* B, float* C, float alpha, float b
                   const tccg_gett_plan* plan, void* stream);
 int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float beta,
                    const tccg_gett_plan* plan, void* stream);
+
+/*
+ * Packed GETT: the reference's pack-then-macro-kernel structure (packA/packB
+ * into dense buffers, tccg/gett.py:346-391 and tccg/gett.py:1092-1098, then
+ * the macro-kernel, tccg/gett.py:229-307) with the packing done once per
+ * operand.  A pack kernel gathers each operand along its stride-one index,
+ * converts it to the MFMA's arithmetic type (for TCCG_BF16X3 the three BF16
+ * parts, for TCCG_FP8 per-tensor scaled e4m3) and writes tile-major planes
+ * into `workspace`; the GETT kernel then copies contiguous planes instead of
+ * gathering and converting every tile.  Same result as tccg_gett_<prec>.
+ *
+ * The plan is a tccg_gett_plan whose `tile` indexes the packed tile table
+ * (tccg_packed_tile_shape()); Kp must be a multiple of 32 and the offset
+ * tables are those of the unpacked path (ordA/ordB are not read).  modeA /
+ * modeB only select the read order of the pack kernel: TCCG_KDIM4/KDIM1 read
+ * along k, TCCG_FREE4/FREE1 along the free set.  `workspace` is device
+ * memory of at least tccg_packed_workspace_bytes() bytes, 16-byte aligned,
+ * owned by the caller and not shared by concurrent calls.  TCCG_FP8 reads the
+ * plan's amax as tccg_gett_fp8 does.
+ */
+size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan);
+int tccg_packed_tile_shape(int prec, int tile, int* BM, int* BN, int* BK);
+int tccg_num_packed_tiles(int prec);
+int tccg_gett_packed_f32(const float* A, const float* B, float* C, float alpha, float beta,
+                         const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+int tccg_gett_packed_f64(const double* A, const double* B, double* C, double alpha, double beta,
+                         const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+int tccg_gett_packed_bf16x3(const float* A, const float* B, float* C, float alpha, float beta,
+                            const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+int tccg_gett_packed_bf16(const float* A, const float* B, float* C, float alpha, float beta,
+                          const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+int tccg_gett_packed_fp8(const float* A, const float* B, float* C, float alpha, float beta,
+                         const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Batched GETT: many independent FP32 contractions in ONE persistent launch.

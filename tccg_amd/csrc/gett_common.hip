@@ -56,6 +56,27 @@ int tccg_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
   return 0;
 }
 
+int tccg_num_packed_tiles(int prec) {
+  int n = 0;
+  tccg_lib::packed_tiles_of(prec, &n);
+  return n;
+}
+
+int tccg_packed_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
+  int n = 0;
+  const tccg_lib::TileCfg* t = tccg_lib::packed_tiles_of(prec, &n);
+  if (!t) return tccg_lib::fail(1, "unknown precision");
+  if (tile < 0 || tile >= n) return tccg_lib::fail(1, "invalid packed tile id");
+  if (BM) *BM = t[tile].BM;
+  if (BN) *BN = t[tile].BN;
+  if (BK) *BK = t[tile].BK;
+  return 0;
+}
+
+size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan) {
+  return tccg_lib::packed_workspace_bytes(prec, plan, nullptr);
+}
+
 int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream) {
   using namespace tccg_lib;
   if (!x || !out || n <= 0) return fail(1, "amax: bad arguments");

@@ -54,7 +54,10 @@ enum Prec : int { F32 = 0, F64 = 1, BF16X3 = 2, FP8 = 3, BF16 = 4 };
 //   KDIM4 : stride-one index is the first index of the k set, extent % 4 == 0
 //           -> vector loads of 4 consecutive k, lanes along k.
 //   KDIM1 : lanes along k, scalar loads.
-enum LoadMode : int { FREE4 = 0, FREE1 = 1, KDIM4 = 2, KDIM1 = 3 };
+//   PACKED: the operand was converted once into tile-major planes by
+//           pack_kernel (gett_pack.hpp); each k-step of a tile is one
+//           contiguous block copied with 16-byte loads, lanes consecutive.
+enum LoadMode : int { FREE4 = 0, FREE1 = 1, KDIM4 = 2, KDIM1 = 3, PACKED = 4 };
 
 // Table entry marking a position that holds no tensor element (padding of a
 // blocked linearisation).  Loads from it read zero; C is not written there.
@@ -220,14 +223,15 @@ struct TileLoader {
   }
 
   // Free-index offsets of this thread's micro-blocks in the tile starting at f0.
-  __device__ inline void tile_offsets(const uint32_t* __restrict__ offF, int f0, int t, uint32_t* out) const {
+  __device__ inline void tile_offsets(const uint32_t* __restrict__ offF, int f0, int t, uint32_t* out, int) const {
 #pragma unroll
     for (int p = 0; p < PT; ++p) out[p] = active(t, p) ? offF[f0 + row[p][0]] : kInvalid;
   }
 
-  __device__ inline void init(int t, const uint32_t* __restrict__ offF, const int* __restrict__ ord, int f0) {
+  __device__ inline void init(int t, const uint32_t* __restrict__ offF, const int* __restrict__ ord, int f0,
+                              int kTiles) {
     init_rows(t, ord);
-    tile_offsets(offF, f0, t, foff);
+    tile_offsets(offF, f0, t, foff, kTiles);
   }
 
   // Table entries of the k-step starting at k0 (issued one step ahead so that
@@ -290,6 +294,73 @@ struct TileLoader {
       }
 #pragma unroll
       for (int e = 0; e < VF; ++e) RowWriter<P>::write(lds, plane_stride, row[p][e] * LDK + kl[p], v[p][e], scale);
+    }
+  }
+};
+
+// Loader of a packed operand.  pack_kernel (gett_pack.hpp) has written the
+// operand as tile-major planes: the k-step kb of tile row block fb occupies
+// BLOCK = PLANES * F * BK consecutive TL elements starting at
+// (fb * kTiles + kb) * BLOCK, ordered [plane][row][k].  A k-step is therefore
+// one contiguous copy (16-byte chunks, consecutive lanes on consecutive
+// chunks), already split or quantised, so no offset tables are read and no
+// conversion is done per tile.
+template <int P, int F, int BK>
+struct TileLoader<P, F, BK, PACKED> {
+  using TR = PrecTraits<P>;
+  using TG = typename TR::TG;
+  using TL = typename TR::TL;
+  static constexpr int VE = 16 / (int)sizeof(TL);       // elements per chunk
+  static constexpr int BLOCK = TR::PLANES * F * BK;     // elements per k-step
+  static constexpr int NCH = BLOCK / VE;
+  static constexpr int PT = (NCH + 255) / 256;
+  static constexpr int NJ = 1;
+  static constexpr int LDK = BK + TR::PAD;
+  static_assert(BK % VE == 0, "a packed row holds whole 16-byte chunks");
+  static_assert((LDK * (int)sizeof(TL)) % 16 == 0, "16-byte aligned LDS rows");
+
+  i32x4 v[PT];
+  uint32_t foff[PT];    // element offset of the tile's first k-step
+  uint32_t kn[PT][NJ];  // element offset of the next k-step within the tile
+
+  __device__ static constexpr bool active(int t, int p) { return (NCH % 256 == 0) || (t + 256 * p) < NCH; }
+
+  __device__ inline void init_rows(int, const int*) {}
+
+  __device__ inline void tile_offsets(const uint32_t*, int f0, int, uint32_t* out, int kTiles) const {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) out[p] = (uint32_t)(f0 / F) * (uint32_t)kTiles * (uint32_t)BLOCK;
+  }
+
+  __device__ inline void init(int t, const uint32_t* offF, const int* ord, int f0, int kTiles) {
+    init_rows(t, ord);
+    tile_offsets(offF, f0, t, foff, kTiles);
+  }
+
+  __device__ inline void fetch_k(const uint32_t*, int k0, int) {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) kn[p][0] = (uint32_t)(k0 / BK) * (uint32_t)BLOCK;
+  }
+
+  __device__ inline void load(const TG* __restrict__ X, int t) {
+    const TL* base = reinterpret_cast<const TL*>(X);
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      if (!active(t, p)) continue;
+      const size_t e = (size_t)foff[p] + kn[p][0] + (size_t)(t + 256 * p) * VE;
+      v[p] = *reinterpret_cast<const i32x4*>(base + e);
+    }
+  }
+
+  __device__ inline void store(TL* lds, int plane_stride, float, int t) {
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      if (!active(t, p)) continue;
+      const int c = t + 256 * p;
+      const int plane = c / (F * BK / VE);
+      const int rem = c % (F * BK / VE);
+      const int r = rem / (BK / VE), kc = rem % (BK / VE);
+      *reinterpret_cast<i32x4*>(lds + plane * plane_stride + r * LDK + kc * VE) = v[p];
     }
   }
 };
@@ -454,8 +525,8 @@ __device__ inline void gett_tile(const GettParams<typename PrecTraits<P>::TG>& p
 
   TileLoader<P, BM, BK, MODEA> la;
   TileLoader<P, BN, BK, MODEB> lb;
-  la.init(t, p.offAm, p.ordA, m0);
-  lb.init(t, p.offBn, p.ordB, n0);
+  la.init(t, p.offAm, p.ordA, m0, p.kTiles);
+  lb.init(t, p.offBn, p.ordB, n0, p.kTiles);
 
   typename MM::Acc acc[S::SN][S::SM];
 #pragma unroll
@@ -595,8 +666,8 @@ __device__ inline void gett_walk(const GettParams<typename PrecTraits<P>::TG>& p
   lb.init_rows(t, p.ordB);
   int tm, tn;
   tile_coords(xcd_chunked(v, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
-  la.tile_offsets(p.offAm, tm * BM, t, la.foff);
-  lb.tile_offsets(p.offBn, tn * BN, t, lb.foff);
+  la.tile_offsets(p.offAm, tm * BM, t, la.foff, KT);
+  lb.tile_offsets(p.offBn, tn * BN, t, lb.foff, KT);
   la.fetch_k(p.offAk, 0, t);
   lb.fetch_k(p.offBk, 0, t);
   // k-step 0 table entries are the same for every tile
@@ -621,8 +692,8 @@ __device__ inline void gett_walk(const GettParams<typename PrecTraits<P>::TG>& p
     uint32_t nfA[decltype(la)::PT], nfB[decltype(lb)::PT];
     if (has_next) {
       tile_coords(xcd_chunked(vn, nblocks), p.tilesM, p.tilesN, p.groupM, tm2, tn2);
-      la.tile_offsets(p.offAm, tm2 * BM, t, nfA);
-      lb.tile_offsets(p.offBn, tn2 * BN, t, nfB);
+      la.tile_offsets(p.offAm, tm2 * BM, t, nfA, KT);
+      lb.tile_offsets(p.offBn, tn2 * BN, t, nfB, KT);
     }
     if (KT > 1) {
       la.fetch_k(p.offAk, BK, t);

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "gett_core.hpp"
+#include "gett_pack.hpp"
 #include "../../include/tccg_gfx950.h"
 
 namespace tccg_lib {
@@ -22,6 +23,43 @@ constexpr TileCfg kF64Tiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}, {192,
 constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192, 128, 128, 2, 2}};
+
+// Tile configurations of the packed path (operands converted once by
+// pack_kernel, then copied tile by tile); ids index these arrays.  Every BM
+// and BN is a multiple of the 64-row pack block.
+constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2}};
+constexpr TileCfg kF64PackedTiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}};
+constexpr TileCfg kBfPackedTiles[] = {{128, 128, 16, 2, 2}, {128, 128, 32, 2, 2}, {256, 128, 16, 2, 2},
+                                      {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2}};
+constexpr TileCfg kBf1PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 32, 2, 2}, {128, 256, 32, 2, 2},
+                                       {64, 64, 32, 2, 2}};
+constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2}};
+
+constexpr int packed_count(int prec) {
+  return prec == tccg::F32 ? (int)(sizeof(kF32PackedTiles) / sizeof(TileCfg))
+       : prec == tccg::F64 ? (int)(sizeof(kF64PackedTiles) / sizeof(TileCfg))
+       : prec == tccg::BF16X3 ? (int)(sizeof(kBfPackedTiles) / sizeof(TileCfg))
+       : prec == tccg::FP8 ? (int)(sizeof(kF8PackedTiles) / sizeof(TileCfg))
+       : prec == tccg::BF16 ? (int)(sizeof(kBf1PackedTiles) / sizeof(TileCfg)) : 0;
+}
+
+constexpr TileCfg packed_tile(int prec, int i) {
+  return prec == tccg::F32 ? kF32PackedTiles[i]
+       : prec == tccg::F64 ? kF64PackedTiles[i]
+       : prec == tccg::BF16X3 ? kBfPackedTiles[i]
+       : prec == tccg::FP8 ? kF8PackedTiles[i] : kBf1PackedTiles[i];
+}
+
+inline const TileCfg* packed_tiles_of(int prec, int* n) {
+  switch (prec) {
+    case tccg::F32: *n = (int)(sizeof(kF32PackedTiles) / sizeof(TileCfg)); return kF32PackedTiles;
+    case tccg::F64: *n = (int)(sizeof(kF64PackedTiles) / sizeof(TileCfg)); return kF64PackedTiles;
+    case tccg::BF16X3: *n = (int)(sizeof(kBfPackedTiles) / sizeof(TileCfg)); return kBfPackedTiles;
+    case tccg::FP8: *n = (int)(sizeof(kF8PackedTiles) / sizeof(TileCfg)); return kF8PackedTiles;
+    case tccg::BF16: *n = (int)(sizeof(kBf1PackedTiles) / sizeof(TileCfg)); return kBf1PackedTiles;
+    default: *n = 0; return nullptr;
+  }
+}
 
 inline const TileCfg* tiles_of(int prec, int* n) {
   switch (prec) {
@@ -57,7 +95,8 @@ int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, in
   const int nblocks = prm.tilesM * prm.tilesN;
   // The persistent walk is built for the vector load modes only; plans with a
   // scalar mode run one tile per workgroup.
-  constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4) && (MB == tccg::FREE4 || MB == tccg::KDIM4);
+  constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4 || MA == tccg::PACKED) &&
+                        (MB == tccg::FREE4 || MB == tccg::KDIM4 || MB == tccg::PACKED);
   if constexpr (kVec) {
     if (grid > 0 && grid < nblocks) return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, true>(prm, grid, stream);
   }
@@ -131,5 +170,131 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   return dispatch_tile<P>(plan->tile, plan->modeA, plan->modeB, prm, plan->grid, (hipStream_t)stream);
 }
 
+
+// Bytes of the packed operands of a plan: A's planes, padded to 256 bytes,
+// then B's planes.  0 if the plan's shape is not one of the packed tiles.
+inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB) {
+  int ntiles = 0;
+  const TileCfg* tiles = packed_tiles_of(prec, &ntiles);
+  if (!tiles || !plan || plan->tile < 0 || plan->tile >= ntiles || plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0)
+    return 0;
+  size_t esz = 4, planes = 1;
+  switch (prec) {
+    case tccg::F64: esz = 8; break;
+    case tccg::BF16X3: esz = 2; planes = 3; break;
+    case tccg::BF16: esz = 2; break;
+    case tccg::FP8: esz = 1; break;
+    default: break;
+  }
+  const size_t a = (planes * (size_t)plan->Mp * (size_t)plan->Kp * esz + 255) & ~(size_t)255;
+  const size_t b = planes * (size_t)plan->Np * (size_t)plan->Kp * esz;
+  if (offB) *offB = a;
+  return a + b;
+}
+
+// GETT kernel over packed operands for packed tile id `tile` (I walks the table).
+template <int P, int I = 0>
+int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
+                         hipStream_t s) {
+  if constexpr (I >= packed_count(P)) {
+    return fail(1, "invalid packed tile id");
+  } else {
+    constexpr TileCfg c = packed_tile(P, I);
+    if (tile == I) return launch_one<P, c.BM, c.BN, c.BK, c.WM, c.WN, tccg::PACKED, tccg::PACKED>(prm, grid, s);
+    return dispatch_packed_tile<P, I + 1>(tile, prm, grid, s);
+  }
+}
+
+template <int P, int F, int BK>
+int launch_pack(const typename tccg::PrecTraits<P>::TG* X, const uint32_t* offF, const uint32_t* offK, int Fp,
+                int Kp, int kfast, const float* amax, typename tccg::PrecTraits<P>::TL* out, hipStream_t s) {
+  dim3 grid(Fp / tccg::kPackTF, Kp / tccg::kPackTK);
+  hipLaunchKernelGGL((tccg::pack_kernel<P, F, BK>), grid, dim3(256), 0, s, X, offF, offK, Kp / BK, kfast, amax, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("pack launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Packed GETT: pack A, pack B (each once, into `workspace`), then the GETT
+// kernel over the packed planes.  All three launches go to `stream`.
+template <int P, int BM, int BN, int BK>
+int run_packed_shape(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
+                     const tccg_gett_plan* plan, typename tccg::PrecTraits<P>::TL* wA,
+                     typename tccg::PrecTraits<P>::TL* wB, hipStream_t s) {
+  const int kfA = (plan->modeA == TCCG_KDIM4 || plan->modeA == TCCG_KDIM1) ? 1 : 0;
+  const int kfB = (plan->modeB == TCCG_KDIM4 || plan->modeB == TCCG_KDIM1) ? 1 : 0;
+  int rc = launch_pack<P, BM, BK>(A, plan->offAm, plan->offAk, plan->Mp, plan->Kp, kfA, plan->amax, wA, s);
+  if (rc) return rc;
+  return launch_pack<P, BN, BK>(B, plan->offBn, plan->offBk, plan->Np, plan->Kp, kfB,
+                                plan->amax ? plan->amax + 1 : nullptr, wB, s);
+}
+
+template <int P, int I = 0>
+int pack_operands(int tile, const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
+                  const tccg_gett_plan* plan, typename tccg::PrecTraits<P>::TL* wA,
+                  typename tccg::PrecTraits<P>::TL* wB, hipStream_t s) {
+  if constexpr (I >= packed_count(P)) {
+    return fail(1, "invalid packed tile id");
+  } else {
+    constexpr TileCfg c = packed_tile(P, I);
+    if (tile == I) return run_packed_shape<P, c.BM, c.BN, c.BK>(A, B, plan, wA, wB, s);
+    return pack_operands<P, I + 1>(tile, A, B, plan, wA, wB, s);
+  }
+}
+
+template <int P>
+int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
+                    typename tccg::PrecTraits<P>::TG* C, typename tccg::PrecTraits<P>::TG alpha,
+                    typename tccg::PrecTraits<P>::TG beta, const tccg_gett_plan* plan, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+  using TG = typename tccg::PrecTraits<P>::TG;
+  using TL = typename tccg::PrecTraits<P>::TL;
+  if (!plan) return fail(1, "plan is NULL");
+  if (!A || !B || !C) return fail(1, "tensor pointer is NULL");
+  if (!workspace) return fail(1, "workspace is NULL");
+  if (!aligned16(workspace)) return fail(1, "workspace must be 16-byte aligned");
+  if (plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0) return fail(1, "table lengths must be positive");
+  int ntiles = 0;
+  const TileCfg* tiles = packed_tiles_of(P, &ntiles);
+  if (plan->tile < 0 || plan->tile >= ntiles) return fail(1, "invalid packed tile id");
+  if (plan->modeA < 0 || plan->modeA > 3 || plan->modeB < 0 || plan->modeB > 3)
+    return fail(1, "invalid read order (load mode)");
+  if (!plan->offAm || !plan->offAk || !plan->offBn || !plan->offBk || !plan->offCm || !plan->offCn)
+    return fail(1, "offset table pointer is NULL");
+  const TileCfg& tc = tiles[plan->tile];
+  if (plan->Mp % tc.BM || plan->Np % tc.BN || plan->Kp % tc.BK)
+    return fail(1, "table lengths must be whole multiples of the tile shape");
+  if (plan->Kp % tccg::kPackTK) return fail(1, "packed plans need Kp to be a multiple of 32");
+  size_t offB = 0;
+  const size_t need = packed_workspace_bytes(P, plan, &offB);
+  if (workspace_bytes < need) return fail(1, "workspace too small for the packed operands");
+  // the PACKED loader addresses elements with 32-bit offsets
+  if ((need - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
+    return fail(1, "packed operand exceeds 2^32-1 elements");
+  if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
+  if (plan->grid < 0) return fail(1, "grid must be >= 0");
+  TL* wA = reinterpret_cast<TL*>(workspace);
+  TL* wB = reinterpret_cast<TL*>(reinterpret_cast<unsigned char*>(workspace) + offB);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
+  if (rc) return rc;
+  tccg::GettParams<TG> prm;
+  prm.A = reinterpret_cast<const TG*>(wA);
+  prm.B = reinterpret_cast<const TG*>(wB);
+  prm.C = C;
+  prm.offAm = plan->offAm; prm.offAk = plan->offAk;
+  prm.offBn = plan->offBn; prm.offBk = plan->offBk;
+  prm.offCm = plan->offCm; prm.offCn = plan->offCn;
+  prm.ordA = nullptr; prm.ordB = nullptr;
+  prm.tilesM = plan->Mp / tc.BM;
+  prm.tilesN = plan->Np / tc.BN;
+  prm.kTiles = plan->Kp / tc.BK;
+  prm.groupM = plan->groupM > 0 ? plan->groupM : 1;
+  prm.alpha = alpha; prm.beta = beta;
+  prm.amax = plan->amax;
+  const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
+  if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
+  return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
+}
 
 }  // namespace tccg_lib

@@ -1,0 +1,10 @@
+// Packed-operand GETT, split-BF16 (three parts): both operands are converted once into
+// tile-major planes in the caller's workspace, then the GETT kernel copies
+// them tile by tile (gett_pack.hpp, PACKED loader of gett_core.hpp).
+#include "gett_launch.hpp"
+
+extern "C" int tccg_gett_packed_bf16x3(const float* A, const float* B, float* C, float alpha, float beta,
+                                    const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes,
+                                    void* stream) {
+  return tccg_lib::run_gett_packed<tccg::BF16X3>(A, B, C, alpha, beta, plan, workspace, workspace_bytes, stream);
+}

@@ -45,6 +45,18 @@ TILES: Dict[int, List[Tuple[int, int, int]]] = {
     FP8: [(128, 128, 128), (64, 64, 128), (192, 128, 128)],
     BF16: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
 }
+# Tile shapes of the packed path (operands converted once into tile-major
+# planes by the pack kernel); must match csrc/gett_launch.hpp's packed tables.
+PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
+    F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16)],
+    F64: [(128, 128, 16), (64, 64, 16)],
+    BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32)],
+    FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128)],
+    BF16: [(128, 128, 32), (256, 128, 32), (128, 256, 32), (64, 64, 32)],
+}
+PACK_TK = 32      # k positions per pack block: packed plans pad Kp to a multiple
+_ELEM_BYTES = {F32: 4, F64: 8, BF16X3: 2, FP8: 1, BF16: 2}
+
 # Wave arrangement (WM x WN) of each tile; (2, 2) unless listed.
 WAVES: Dict[Tuple[int, int], Tuple[int, int]] = {(F32, 7): (1, 4)}
 
@@ -275,6 +287,7 @@ class GettPlan:
     modeB: int
     groupM: int = 8
     grid: int = 0          # 0: one workgroup per tile; > 0: persistent walk
+    packed: bool = False   # operands converted once into tile-major planes
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -295,10 +308,18 @@ class GettPlan:
     def padded_flops(self) -> float:
         return 2.0 * self.Mp * self.Np * self.Kp
 
+    def workspace_bytes(self) -> int:
+        """Device bytes of the packed operands (0 for an unpacked plan); the
+        same figure as tccg_packed_workspace_bytes()."""
+        if not self.packed:
+            return 0
+        per = _PLANES[self.prec] * _ELEM_BYTES[self.prec] * self.Kp
+        return (-(-(per * self.Mp) // 256) * 256) + per * self.Np
+
     def describe(self) -> str:
         mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
-        return ("prec=%s tile=%d (%dx%dx%d)%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
-                % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK,
+        return ("prec=%s tile=%d (%dx%dx%d)%s%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
+                % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, " packed" if self.packed else "",
                    (" persistent grid=%d" % self.grid) if self.grid else "", self.lin_m.inner,
                    self.lin_n.inner, self.lin_k.inner, mode[self.modeA], mode[self.modeB], self.ntiles(),
                    self.Mp, self.Np, self.Kp))
@@ -335,6 +356,41 @@ def _ext(pb: GettProblem, labels: Sequence[str]) -> List[int]:
                 out.append(T.indices[p].size)
                 break
     return out
+
+
+def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
+                      persistent: bool = False) -> GettPlan:
+    """Plan of the packed path: the reference's pack-then-macro-kernel GETT
+    (tccg/gett.py:346-391, tccg/gett.py:1092-1098) with each operand packed
+    once.  Plain linearisations (the pack kernel reads every element once, so
+    no blocking is needed for the operand loads); m in C's order so C is
+    written along its stride-one index; Kp padded to the pack block.  modeA /
+    modeB hold the pack kernel's read order: KDIM1 when the operand's
+    stride-one index is contracted, else FREE1."""
+    BM, BN, BK = PACKED_TILES[prec][tile]
+    m_l, n_l, k_l = _orders(pb)
+    lin_m = Linearization.plain(m_l, _ext(pb, m_l))
+    lin_n = Linearization.plain(n_l, _ext(pb, n_l))
+    lin_k = Linearization.plain(k_l, _ext(pb, k_l))
+    kpad = BK * PACK_TK // math.gcd(BK, PACK_TK)
+    offAm = lin_m.offsets(pb.opA, BM)
+    offCm = lin_m.offsets(pb.C, BM)
+    offBn = lin_n.offsets(pb.opB, BN)
+    offCn = lin_n.offsets(pb.C, BN)
+    offAk = lin_k.offsets(pb.opA, kpad)
+    offBk = lin_k.offsets(pb.opB, kpad)
+    modeA = KDIM1 if pb.opA.indices[0].label in k_l else FREE1
+    modeB = KDIM1 if pb.opB.indices[0].label in k_l else FREE1
+    ident_m = np.arange(BM, dtype=np.int32)
+    ident_n = np.arange(BN, dtype=np.int32)
+    plan = GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
+                    ident_m, ident_n, modeA, modeB, groupM, packed=True)
+    if persistent:
+        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // lds_bytes(prec, BM, BN, BK)))
+        plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
+        if plan.grid >= plan.ntiles():
+            plan.grid = 0
+    return plan
 
 
 def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] = None,
@@ -410,10 +466,21 @@ def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
     slots = hw["cus"] * per_cu
     ntiles = plan.ntiles()
     eff = 0.75 if plan.BM * plan.BN >= 128 * 128 else 0.55
-    if plan.modeA in (FREE1, KDIM1):
-        eff *= 0.85
-    if plan.modeB in (FREE1, KDIM1):
-        eff *= 0.85
+    t_pack = 0.0
+    if plan.packed:
+        # contiguous tile copies, no per-tile conversion; the pack kernels read
+        # each operand once and write its planes once
+        eff = 0.85 if plan.BM * plan.BN >= 128 * 128 else 0.6
+        src = 8 if plan.prec == F64 else 4
+        moved = (src + _PLANES[plan.prec] * _ELEM_BYTES[plan.prec]) * (plan.Mp + plan.Np) * plan.Kp
+        t_pack = 6.0e-6 + moved / (0.6 * hw["hbm_bytes_per_s"])
+    else:
+        if plan.modeA in (FREE1, KDIM1):
+            eff *= 0.85
+        if plan.modeB in (FREE1, KDIM1):
+            eff *= 0.85
+        # operands split or quantised again for every tile that reads them
+        eff *= {BF16X3: 0.45, BF16: 0.6, FP8: 0.6}.get(plan.prec, 1.0)
     t_mfma = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots) / eff
     esz = 8 if plan.prec == F64 else 4
     t_store = esz * plan.BM * plan.BN / (0.7 * hw["hbm_bytes_per_s"] / slots)
@@ -421,7 +488,7 @@ def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
     if plan.grid:
         t_fixed = 0.4e-6 + t_store * 0.6
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
-    t = rounds * (t_mfma + t_fixed)
+    t = rounds * (t_mfma + t_fixed) + t_pack
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
     t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
     return max(t, t_hbm)
@@ -441,6 +508,12 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
                 pp = build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM, persistent=True)
                 if pp.grid:
                     plans.append(pp)
+    for tile in range(len(PACKED_TILES[prec])):
+        plans.append(build_packed_plan(pb, prec, tile, groupM=groupM))
+        if pb.K <= 256:
+            pp = build_packed_plan(pb, prec, tile, groupM=groupM, persistent=True)
+            if pp.grid:
+                plans.append(pp)
     plans.sort(key=lambda p: estimate_seconds(p))
     return plans
 

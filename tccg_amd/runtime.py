@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 from . import build as _build
-from .plan import BF16, BF16X3, F32, F64, FP8, GettPlan, TILES
+from .plan import BF16, BF16X3, F32, F64, FP8, PACKED_TILES, GettPlan, TILES
 
 
 class GettRuntimeError(RuntimeError):
@@ -38,7 +38,13 @@ class _CPlan(ctypes.Structure):
 
 EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_gett_f64",
                     "tccg_gett_bf16x3", "tccg_gett_fp8", "tccg_gett_bf16", "tccg_amax_f32", "tccg_last_error",
-                    "tccg_batch_num_variants", "tccg_batch_variant", "tccg_gett_batch_f32"]
+                    "tccg_batch_num_variants", "tccg_batch_variant", "tccg_gett_batch_f32",
+                    "tccg_packed_workspace_bytes", "tccg_packed_tile_shape", "tccg_num_packed_tiles",
+                    "tccg_gett_packed_f32", "tccg_gett_packed_f64", "tccg_gett_packed_bf16x3",
+                    "tccg_gett_packed_bf16", "tccg_gett_packed_fp8"]
+
+_PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
+              BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -68,22 +74,34 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
             fn = getattr(lib, name)
             fn.argtypes = [vp, vp, vp, T, T, ctypes.POINTER(_CPlan), vp]
             fn.restype = ctypes.c_int
+        for prec, name in _PACKED_FN.items():
+            fn = getattr(lib, name)
+            T = f64 if prec == F64 else f32
+            fn.argtypes = [vp, vp, vp, T, T, ctypes.POINTER(_CPlan), vp, ctypes.c_size_t, vp]
+            fn.restype = ctypes.c_int
+        lib.tccg_packed_workspace_bytes.argtypes = [ctypes.c_int, ctypes.POINTER(_CPlan)]
+        lib.tccg_packed_workspace_bytes.restype = ctypes.c_size_t
+        lib.tccg_packed_tile_shape.argtypes = lib.tccg_tile_shape.argtypes
+        lib.tccg_num_packed_tiles.argtypes = [ctypes.c_int]
         lib.tccg_amax_f32.argtypes = [vp, ctypes.c_int64, vp, vp]
         lib.tccg_gett_batch_f32.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
         lib.tccg_gett_batch_f32.restype = ctypes.c_int
         lib.tccg_batch_variant.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 5
         lib.tccg_last_error.restype = ctypes.c_char_p
-        # The Python tile table must agree with the library's.
-        for prec, tiles in TILES.items():
-            n = lib.tccg_num_tiles(prec)
-            if n != len(tiles):
-                raise GettRuntimeError("tile table mismatch for precision %d: library %d, host %d"
-                                       % (prec, n, len(tiles)))
-            for t, shape in enumerate(tiles):
-                bm, bn, bk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-                lib.tccg_tile_shape(prec, t, ctypes.byref(bm), ctypes.byref(bn), ctypes.byref(bk))
-                if (bm.value, bn.value, bk.value) != shape:
-                    raise GettRuntimeError("tile %d/%d shape mismatch" % (prec, t))
+        # The Python tile tables must agree with the library's.
+        for table, count, shape_of, kind in ((TILES, lib.tccg_num_tiles, lib.tccg_tile_shape, "tile"),
+                                              (PACKED_TILES, lib.tccg_num_packed_tiles, lib.tccg_packed_tile_shape,
+                                               "packed tile")):
+            for prec, tiles in table.items():
+                n = count(prec)
+                if n != len(tiles):
+                    raise GettRuntimeError("%s table mismatch for precision %d: library %d, host %d"
+                                           % (kind, prec, n, len(tiles)))
+                for t, shape in enumerate(tiles):
+                    bm, bn, bk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+                    shape_of(prec, t, ctypes.byref(bm), ctypes.byref(bn), ctypes.byref(bk))
+                    if (bm.value, bn.value, bk.value) != shape:
+                        raise GettRuntimeError("%s %d/%d shape mismatch" % (kind, prec, t))
         _lib = lib
         return lib
 
@@ -127,6 +145,12 @@ class DevicePlan:
         c.grid = plan.grid
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
+        self.workspace = None
+        if plan.packed:
+            nbytes = plan.workspace_bytes()
+            if load_library().tccg_packed_workspace_bytes(plan.prec, ctypes.byref(c)) != nbytes:
+                raise GettRuntimeError("packed workspace size disagrees with the library")
+            self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
 
     def _operands(self, A, B, C):
         """Tensors in the .tccg's (A, B, C) order -> kernel operands (opA, opB)."""
@@ -161,9 +185,14 @@ class DevicePlan:
                 rc = lib.tccg_amax_f32(opB.data_ptr(), opB.numel(), self.amax.data_ptr() + 4, sptr)
             if rc != 0:
                 raise GettRuntimeError(last_error())
-        fn = {F32: lib.tccg_gett_f32, F64: lib.tccg_gett_f64, BF16X3: lib.tccg_gett_bf16x3,
-              FP8: lib.tccg_gett_fp8, BF16: lib.tccg_gett_bf16}[prec]
-        rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
+        if self.plan.packed:
+            fn = getattr(lib, _PACKED_FN[prec])
+            rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan),
+                    self.workspace.data_ptr(), self.workspace.numel(), sptr)
+        else:
+            fn = {F32: lib.tccg_gett_f32, F64: lib.tccg_gett_f64, BF16X3: lib.tccg_gett_bf16x3,
+                  FP8: lib.tccg_gett_fp8, BF16: lib.tccg_gett_bf16}[prec]
+            rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
         if rc != 0:
             raise GettRuntimeError("GETT launch failed: %s" % last_error())
 
