@@ -54,10 +54,9 @@ enum Prec : int { F32 = 0, F64 = 1, BF16X3 = 2, FP8 = 3, BF16 = 4 };
 //   KDIM4 : stride-one index is the first index of the k set, extent % 4 == 0
 //           -> vector loads of 4 consecutive k, lanes along k.
 //   KDIM1 : lanes along k, scalar loads.
-//   PACKED: the operand was converted once into tile-major planes by
-//           pack_kernel (gett_pack.hpp); each k-step of a tile is one
-//           contiguous block copied with 16-byte loads, lanes consecutive.
-enum LoadMode : int { FREE4 = 0, FREE1 = 1, KDIM4 = 2, KDIM1 = 3, PACKED = 4 };
+// (Operands packed once by pack_kernel are read by gett_packed_kernel,
+// gett_packed.hpp, not by this loader.)
+enum LoadMode : int { FREE4 = 0, FREE1 = 1, KDIM4 = 2, KDIM1 = 3 };
 
 // Table entry marking a position that holds no tensor element (padding of a
 // blocked linearisation).  Loads from it read zero; C is not written there.
@@ -298,73 +297,6 @@ struct TileLoader {
   }
 };
 
-// Loader of a packed operand.  pack_kernel (gett_pack.hpp) has written the
-// operand as tile-major planes: the k-step kb of tile row block fb occupies
-// BLOCK = PLANES * F * BK consecutive TL elements starting at
-// (fb * kTiles + kb) * BLOCK, ordered [plane][row][k].  A k-step is therefore
-// one contiguous copy (16-byte chunks, consecutive lanes on consecutive
-// chunks), already split or quantised, so no offset tables are read and no
-// conversion is done per tile.
-template <int P, int F, int BK>
-struct TileLoader<P, F, BK, PACKED> {
-  using TR = PrecTraits<P>;
-  using TG = typename TR::TG;
-  using TL = typename TR::TL;
-  static constexpr int VE = 16 / (int)sizeof(TL);       // elements per chunk
-  static constexpr int BLOCK = TR::PLANES * F * BK;     // elements per k-step
-  static constexpr int NCH = BLOCK / VE;
-  static constexpr int PT = (NCH + 255) / 256;
-  static constexpr int NJ = 1;
-  static constexpr int LDK = BK + TR::PAD;
-  static_assert(BK % VE == 0, "a packed row holds whole 16-byte chunks");
-  static_assert((LDK * (int)sizeof(TL)) % 16 == 0, "16-byte aligned LDS rows");
-
-  i32x4 v[PT];
-  uint32_t foff[PT];    // element offset of the tile's first k-step
-  uint32_t kn[PT][NJ];  // element offset of the next k-step within the tile
-
-  __device__ static constexpr bool active(int t, int p) { return (NCH % 256 == 0) || (t + 256 * p) < NCH; }
-
-  __device__ inline void init_rows(int, const int*) {}
-
-  __device__ inline void tile_offsets(const uint32_t*, int f0, int, uint32_t* out, int kTiles) const {
-#pragma unroll
-    for (int p = 0; p < PT; ++p) out[p] = (uint32_t)(f0 / F) * (uint32_t)kTiles * (uint32_t)BLOCK;
-  }
-
-  __device__ inline void init(int t, const uint32_t* offF, const int* ord, int f0, int kTiles) {
-    init_rows(t, ord);
-    tile_offsets(offF, f0, t, foff, kTiles);
-  }
-
-  __device__ inline void fetch_k(const uint32_t*, int k0, int) {
-#pragma unroll
-    for (int p = 0; p < PT; ++p) kn[p][0] = (uint32_t)(k0 / BK) * (uint32_t)BLOCK;
-  }
-
-  __device__ inline void load(const TG* __restrict__ X, int t) {
-    const TL* base = reinterpret_cast<const TL*>(X);
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {
-      if (!active(t, p)) continue;
-      const size_t e = (size_t)foff[p] + kn[p][0] + (size_t)(t + 256 * p) * VE;
-      v[p] = *reinterpret_cast<const i32x4*>(base + e);
-    }
-  }
-
-  __device__ inline void store(TL* lds, int plane_stride, float, int t) {
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {
-      if (!active(t, p)) continue;
-      const int c = t + 256 * p;
-      const int plane = c / (F * BK / VE);
-      const int rem = c % (F * BK / VE);
-      const int r = rem / (BK / VE), kc = rem % (BK / VE);
-      *reinterpret_cast<i32x4*>(lds + plane * plane_stride + r * LDK + kc * VE) = v[p];
-    }
-  }
-};
-
 // ---------------------------------------------------------------------------
 // MFMA micro-tile policies.  Fragments are read from k-contiguous LDS rows;
 // `b` is the n-operand (MFMA rows), `a` the m-operand (MFMA columns).
@@ -505,6 +437,54 @@ struct TileShape {
   static_assert(BK % TR::KF == 0, "k tile");
 };
 
+// Epilogue of one tile: alpha/beta update of C along its stride-one index.
+// The C offsets of this lane's rows and columns are gathered before any store.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& p,
+                                  typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                             [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                  int m0, int n0, int wm, int wn, int lane, float descale) {
+  using TR = PrecTraits<P>;
+  using TG = typename TR::TG;
+  using S = TileShape<P, BM, BN, BK, WM, WN>;
+  using MM = Mma<P>;
+  const TG alpha = p.alpha, beta = p.beta;
+  uint32_t cm[S::SM], cn[S::SN][MM::NACC];
+#pragma unroll
+  for (int j = 0; j < S::SM; ++j) {
+    int n_l, m_l;
+    MM::coord(lane, 0, n_l, m_l);
+    cm[j] = p.offCm[m0 + wm * S::WTM + j * TR::SUB + m_l];
+  }
+#pragma unroll
+  for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+    for (int r = 0; r < MM::NACC; ++r) {
+      int n_l, m_l;
+      MM::coord(lane, r, n_l, m_l);
+      cn[i][r] = p.offCn[n0 + wn * S::WTN + i * TR::SUB + n_l];
+    }
+#pragma unroll
+  for (int j = 0; j < S::SM; ++j) {
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i) {
+#pragma unroll
+      for (int r = 0; r < MM::NACC; ++r) {
+        if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
+          TG val = (TG)acc[i][j][r];
+          if (P == FP8) val = (TG)((float)val * descale);
+          TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
+          if (beta == TG(0)) {
+            __builtin_nontemporal_store(alpha * val, dst);
+          } else {
+            *dst = alpha * val + beta * (*dst);
+          }
+        }
+      }
+    }
+  }
+}
+
 // One BM x BN tile (tm, tn) of C over the whole k range, computed by the 256
 // threads of a workgroup.  `smem` holds two LDS stages (TileShape::LDS_BYTES).
 template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB>
@@ -589,43 +569,7 @@ __device__ inline void gett_tile(const GettParams<typename PrecTraits<P>::TG>& p
     __syncthreads();
   }
 
-  // Epilogue: alpha/beta update of C along its stride-one index.  The C
-  // offsets of this lane's rows and columns are gathered before any store.
-  const TG alpha = p.alpha, beta = p.beta;
-  uint32_t cm[S::SM], cn[S::SN][MM::NACC];
-#pragma unroll
-  for (int j = 0; j < S::SM; ++j) {
-    int n_l, m_l;
-    MM::coord(lane, 0, n_l, m_l);
-    cm[j] = p.offCm[m0 + wm * S::WTM + j * TR::SUB + m_l];
-  }
-#pragma unroll
-  for (int i = 0; i < S::SN; ++i)
-#pragma unroll
-    for (int r = 0; r < MM::NACC; ++r) {
-      int n_l, m_l;
-      MM::coord(lane, r, n_l, m_l);
-      cn[i][r] = p.offCn[n0 + wn * S::WTN + i * TR::SUB + n_l];
-    }
-#pragma unroll
-  for (int j = 0; j < S::SM; ++j) {
-#pragma unroll
-    for (int i = 0; i < S::SN; ++i) {
-#pragma unroll
-      for (int r = 0; r < MM::NACC; ++r) {
-        if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
-          TG val = (TG)acc[i][j][r];
-          if (P == FP8) val = (TG)((float)val * descale);
-          TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
-          if (beta == TG(0)) {
-            __builtin_nontemporal_store(alpha * val, dst);
-          } else {
-            *dst = alpha * val + beta * (*dst);
-          }
-        }
-      }
-    }
-  }
+  store_tile<P, BM, BN, BK, WM, WN>(p, acc, m0, n0, wm, wn, lane, descale);
 }
 
 // A workgroup walks the C tiles blockIdx.x, blockIdx.x + gridDim.x, ... (in

@@ -27,12 +27,13 @@ constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192
 // Tile configurations of the packed path (operands converted once by
 // pack_kernel, then copied tile by tile); ids index these arrays.  Every BM
 // and BN is a multiple of the 64-row pack block.
-constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2}};
+constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
+                                       {192, 192, 16, 2, 2}};
 constexpr TileCfg kF64PackedTiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}};
 constexpr TileCfg kBfPackedTiles[] = {{128, 128, 16, 2, 2}, {128, 128, 32, 2, 2}, {256, 128, 16, 2, 2},
-                                      {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2}};
+                                      {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2},   {192, 192, 16, 2, 2}};
 constexpr TileCfg kBf1PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 32, 2, 2}, {128, 256, 32, 2, 2},
-                                       {64, 64, 32, 2, 2}};
+                                       {64, 64, 32, 2, 2},   {192, 192, 32, 2, 2}};
 constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2}};
 
 constexpr int packed_count(int prec) {
@@ -95,8 +96,7 @@ int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, in
   const int nblocks = prm.tilesM * prm.tilesN;
   // The persistent walk is built for the vector load modes only; plans with a
   // scalar mode run one tile per workgroup.
-  constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4 || MA == tccg::PACKED) &&
-                        (MB == tccg::FREE4 || MB == tccg::KDIM4 || MB == tccg::PACKED);
+  constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4) && (MB == tccg::FREE4 || MB == tccg::KDIM4);
   if constexpr (kVec) {
     if (grid > 0 && grid < nblocks) return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, true>(prm, grid, stream);
   }
@@ -192,7 +192,29 @@ inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_
   return a + b;
 }
 
-// GETT kernel over packed operands for packed tile id `tile` (I walks the table).
+// Stages of the LDS ring of the packed GETT kernel.
+constexpr int kPackedStages = 3;
+
+template <int P, int BM, int BN, int BK, int WM, int WN, bool PERSIST>
+int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
+                         hipStream_t stream) {
+  using SH = tccg::PackedShape<P, BM, BN, BK, kPackedStages>;
+  auto kern = tccg::gett_packed_kernel<P, BM, BN, BK, WM, WN, kPackedStages, PERSIST>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, SH::LDS_BYTES);
+    if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), SH::LDS_BYTES, stream, prm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// GETT kernel over packed operands for packed tile id `tile` (I walks the
+// table); grid > 0 and below the tile count selects the persistent walk.
 template <int P, int I = 0>
 int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
                          hipStream_t s) {
@@ -200,7 +222,11 @@ int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTra
     return fail(1, "invalid packed tile id");
   } else {
     constexpr TileCfg c = packed_tile(P, I);
-    if (tile == I) return launch_one<P, c.BM, c.BN, c.BK, c.WM, c.WN, tccg::PACKED, tccg::PACKED>(prm, grid, s);
+    if (tile == I) {
+      const int nblocks = prm.tilesM * prm.tilesN;
+      if (grid > 0 && grid < nblocks) return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, true>(prm, grid, s);
+      return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, false>(prm, nblocks, s);
+    }
     return dispatch_packed_tile<P, I + 1>(tile, prm, grid, s);
   }
 }
@@ -268,7 +294,7 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   size_t offB = 0;
   const size_t need = packed_workspace_bytes(P, plan, &offB);
   if (workspace_bytes < need) return fail(1, "workspace too small for the packed operands");
-  // the PACKED loader addresses elements with 32-bit offsets
+  // the packed kernels address elements with 32-bit offsets
   if ((need - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
     return fail(1, "packed operand exceeds 2^32-1 elements");
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");

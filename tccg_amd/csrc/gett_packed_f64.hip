@@ -1,6 +1,7 @@
 // Packed-operand GETT, FP64: both operands are converted once into
 // tile-major planes in the caller's workspace, then the GETT kernel copies
-// them tile by tile (gett_pack.hpp, PACKED loader of gett_core.hpp).
+// them through an LDS ring with direct global->LDS copies (gett_pack.hpp,
+// gett_packed.hpp).
 #include "gett_launch.hpp"
 
 extern "C" int tccg_gett_packed_f64(const double* A, const double* B, double* C, double alpha, double beta,
