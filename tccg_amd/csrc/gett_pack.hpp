@@ -11,48 +11,49 @@
 // (gett_packed.hpp) copies into LDS with direct global->LDS loads:
 //
 //   element (f, k) of plane q  ->  (f / F) * kTiles * BLOCK + (k / BK) * BLOCK
-//                                  + q * F * BK + (f % F) * BK + pos(k % BK),
+//                                  + q * F * BK + pos(f % F, k % BK),
 //   BLOCK = PLANES * F * BK,
 //
 // for a tile of F free positions (BM for A, BN for B) by BK k positions,
-// where pos() keeps the element's place inside its 16-byte chunk and moves
-// chunk c of row r to c ^ swz(r) (gett_packed.hpp), the permutation that
-// makes the GETT kernel's LDS fragment reads conflict-free.  Positions whose
-// table entry is the invalid marker hold zero.
+// where pos() (chunk_pos() of gett_packed.hpp plus the place inside the
+// 16-byte chunk) lays each plane out as 32-byte sub-rows with a per-row
+// chunk swap, the layout that keeps the GETT kernel's LDS fragment reads
+// free of bank conflicts.  Positions whose table entry is the invalid
+// marker hold zero.
 #pragma once
 #include "gett_packed.hpp"
 
 namespace tccg {
 
-// Eight consecutive k values (logical offsets kin..kin+7 of one packed row,
-// kin a multiple of 8) into every plane of the packed layout; `row` points
-// at the row in plane 0 and 16-byte chunk c of the row is stored at chunk
-// c ^ sw.
+// Eight consecutive k values (offsets kin..kin+7 of row r inside a k-step,
+// kin a multiple of 8) into every plane of a packed block of F rows starting
+// at `blk`; 16-byte chunk c of the row goes to chunk_pos(F, r, c).
 template <int P> struct PackWriter;
 
 template <> struct PackWriter<F32> {
-  __device__ static inline void write8(float* row, int, int kin, int sw, const float* v, float) {
+  __device__ static inline void write8(float* blk, int F, int, int r, int kin, const float* v, float) {
     const int c = kin >> 2;
     const f32x4 x0 = {v[0], v[1], v[2], v[3]};
     const f32x4 x1 = {v[4], v[5], v[6], v[7]};
-    *reinterpret_cast<f32x4*>(row + ((c ^ sw) << 2)) = x0;
-    *reinterpret_cast<f32x4*>(row + (((c + 1) ^ sw) << 2)) = x1;
+    *reinterpret_cast<f32x4*>(blk + chunk_pos<float>(F, r, c)) = x0;
+    *reinterpret_cast<f32x4*>(blk + chunk_pos<float>(F, r, c + 1)) = x1;
   }
 };
 template <> struct PackWriter<F64> {
-  __device__ static inline void write8(double* row, int, int kin, int sw, const double* v, float) {
+  __device__ static inline void write8(double* blk, int F, int, int r, int kin, const double* v, float) {
     const int c = kin >> 1;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const f64x2 x = {v[2 * j], v[2 * j + 1]};
-      *reinterpret_cast<f64x2*>(row + (((c + j) ^ sw) << 1)) = x;
+      *reinterpret_cast<f64x2*>(blk + chunk_pos<double>(F, r, c + j)) = x;
     }
   }
 };
 // a = a0 + a1 + a2 with each part rounded to nearest even BF16; the three
 // parts represent every FP32 value exactly.
 template <> struct PackWriter<BF16X3> {
-  __device__ static inline void write8(__bf16* row, int plane_stride, int kin, int sw, const float* v, float) {
+  __device__ static inline void write8(__bf16* blk, int F, int plane_stride, int r, int kin, const float* v,
+                                       float) {
     bf16x8 p0, p1, p2;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -65,22 +66,22 @@ template <> struct PackWriter<BF16X3> {
       p1[j] = a1;
       p2[j] = (__bf16)r2;
     }
-    __bf16* dst = row + (((kin >> 3) ^ sw) << 3);
+    __bf16* dst = blk + chunk_pos<__bf16>(F, r, kin >> 3);
     *reinterpret_cast<bf16x8*>(dst) = p0;
     *reinterpret_cast<bf16x8*>(dst + plane_stride) = p1;
     *reinterpret_cast<bf16x8*>(dst + 2 * plane_stride) = p2;
   }
 };
 template <> struct PackWriter<BF16> {
-  __device__ static inline void write8(__bf16* row, int, int kin, int sw, const float* v, float) {
+  __device__ static inline void write8(__bf16* blk, int F, int, int r, int kin, const float* v, float) {
     bf16x8 p0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) p0[j] = (__bf16)v[j];
-    *reinterpret_cast<bf16x8*>(row + (((kin >> 3) ^ sw) << 3)) = p0;
+    *reinterpret_cast<bf16x8*>(blk + chunk_pos<__bf16>(F, r, kin >> 3)) = p0;
   }
 };
 template <> struct PackWriter<FP8> {
-  __device__ static inline void write8(uint8_t* row, int, int kin, int sw, const float* v, float scale) {
+  __device__ static inline void write8(uint8_t* blk, int F, int, int r, int kin, const float* v, float scale) {
     float s[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) s[j] = fminf(fmaxf(v[j] * scale, -448.0f), 448.0f);
@@ -90,7 +91,7 @@ template <> struct PackWriter<FP8> {
     w1 = __builtin_amdgcn_cvt_pk_fp8_f32(s[6], s[7], w1, true);
     typedef int i32x2 __attribute__((ext_vector_type(2)));
     const i32x2 w = {w0, w1};
-    *reinterpret_cast<i32x2*>(row + (((kin >> 4) ^ sw) << 4) + (kin & 15)) = w;
+    *reinterpret_cast<i32x2*>(blk + chunk_pos<uint8_t>(F, r, kin >> 4) + (kin & 15)) = w;
   }
 };
 
@@ -160,10 +161,8 @@ pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __
     scale = 448.f / a;
   }
   const int gf = f0 + f, gk = k0 + kl;
-  const int row = gf % F;
-  constexpr int KC = BK * (int)sizeof(typename TR::TL) / 16;
-  const size_t dst = (size_t)(gf / F) * (size_t)kTiles * BLOCK + (size_t)(gk / BK) * BLOCK + (size_t)row * BK;
-  PackWriter<P>::write8(out + dst, F * BK, gk % BK, swz<KC>(row), v, scale);
+  const size_t blk = (size_t)(gf / F) * (size_t)kTiles * BLOCK + (size_t)(gk / BK) * BLOCK;
+  PackWriter<P>::write8(out + blk, F, F * BK, gf % F, gk % BK, v, scale);
 }
 
 }  // namespace tccg

@@ -2,55 +2,65 @@
 //
 // pack_kernel (gett_pack.hpp) has written each operand once as tile-major
 // blocks: k-step kb of tile row block fb is BLOCK = PLANES * F * BK
-// consecutive elements at (fb * kTiles + kb) * BLOCK, ordered [plane][row][k]
-// with the 16-byte chunks of each row permuted by swz() below.  A k-step of
-// a C tile is therefore two contiguous blocks (A's and B's), which this
-// kernel copies into LDS with direct global->LDS loads
-// (global_load_lds_dwordx4: 64 lanes x 16 bytes = one contiguous kilobyte
-// per instruction, no VGPR staging, no LDS store instructions) into a ring
-// of S stages, S - 1 k-steps ahead of the MFMAs.  One barrier per k-step:
+// consecutive elements at (fb * kTiles + kb) * BLOCK.  Inside a block each
+// plane is split into 32-byte sub-rows, [plane][k / KS][row][KS] with KS the
+// elements of 32 bytes, and the two 16-byte chunks of a sub-row of row r are
+// swapped when swz2(r) = 1 (chunk_pos() below).  A k-step of a C tile is
+// therefore two contiguous blocks (A's and B's), which this kernel copies
+// into LDS with direct global->LDS loads (global_load_lds_dwordx4: 64 lanes x
+// 16 bytes = one contiguous kilobyte per instruction, no VGPR staging, no LDS
+// store instructions) into a ring of S stages, S k-steps ahead of the MFMAs.
+// The MFMA fragments of step s + 1 are read from LDS while the MFMAs of step
+// s run (two register sets).  One barrier per k-step:
 //
-//   wait until this wave's loads of step s have landed (vmcnt)
-//   barrier   -> every wave's loads of step s landed, and every wave is
-//                done reading step s - 1, whose stage is refilled next
-//   issue the loads of step s + S - 1 into stage (s + S - 1) % S
-//   MFMAs of step s from stage s % S
+//   wait: this wave's copies of step s + 1 have landed (vmcnt) and its
+//         fragment reads of step s are done (lgkmcnt)
+//   barrier -> every wave's copies of step s + 1 landed; nobody reads the
+//              stage of step s any more
+//   copy step s + S into the stage of step s
+//   read the fragments of step s + 1; MFMAs of step s
 //
-// The barrier carries no memory fence (a fence would wait for every load in
-// flight, including the prefetches); the vmcnt wait before it is what makes
-// this wave's copies of step s complete.  Compiler-level memory barriers
-// keep LDS reads from moving across it.
+// The barrier carries no memory fence (a fence would wait for every copy in
+// flight, including the prefetches); the explicit wait before it is what
+// makes this wave's copies complete.  Compiler-level memory barriers keep
+// LDS accesses from moving across it.
 //
-// LDS rows are unpadded (the copy is contiguous), so the chunk permutation
-// chosen by the pack kernel is what keeps the MFMA fragment reads free of
-// bank conflicts: the 8 or 16 lanes of one LDS access cycle read the same
-// logical chunk of consecutive rows, and swz() sends those to distinct banks.
+// LDS rows are unpadded (the copy is contiguous).  The layout keeps the MFMA
+// fragment reads free of bank conflicts: the lanes of one LDS access cycle
+// read the same logical chunk of consecutive rows, whose 32-byte sub-rows are
+// adjacent, and the chunk swap by swz2() spreads rows r and r + 4 (r + 8)
+// over the other half of the banks.
 #pragma once
 #include "gett_core.hpp"
 
 namespace tccg {
 
-// Physical position of 16-byte chunk c of packed row r is c ^ swz<KC>(r),
-// KC = chunks per row.
-template <int KC>
-__host__ __device__ constexpr int swz(int r) {
-  return KC == 2 ? (((r >> 2) ^ (r >> 3)) & 1)
-       : KC == 4 ? (((r >> 1) ^ (r >> 3)) & 3)
-       : KC == 8 ? ((r ^ (r >> 3)) & 7) : 0;
+// Element offset, inside one plane of a packed block of F rows, of logical
+// 16-byte chunk c (counted along the k direction) of row r.
+__host__ __device__ constexpr int swz2(int r) { return ((r >> 2) ^ (r >> 3)) & 1; }
+
+template <class TL>
+__host__ __device__ constexpr int chunk_pos(int F, int r, int c) {
+  constexpr int VE = 16 / (int)sizeof(TL), KS = 2 * VE;
+  return (c >> 1) * (F * KS) + r * KS + (((c & 1) ^ swz2(r)) * VE);
 }
 
-// s_waitcnt vmcnt(N) alone (expcnt and lgkmcnt left at their maxima).
+// s_waitcnt vmcnt(N) lgkmcnt(0) (expcnt left at its maximum).
 template <int N>
-__device__ inline void wait_vmcnt() {
+__device__ inline void wait_vm_lgkm0() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 0xf) | (0x7 << 4) | (0xf << 8) | ((N >> 4) << 14));
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | (0x7 << 4) | (0x0 << 8) | ((N >> 4) << 14));
 }
 
-// Workgroup barrier without a memory fence.
+// Workgroup barrier without a memory fence.  The scheduling barriers keep
+// every instruction (MFMAs included) on its side, so the MFMAs of a k-step
+// stay after the barrier that publishes the fragments read for it.
 __device__ inline void block_barrier() {
+  __builtin_amdgcn_sched_barrier(0);
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
 }
 
 template <int P, int BM, int BN, int BK, int S>
@@ -63,56 +73,64 @@ struct PackedShape {
   static constexpr int LDS_BYTES = S * STAGE_BYTES;
   static constexpr int NA = BYTES_A / 1024, NB = BYTES_B / 1024;  // kilobyte copies per k-step
   static constexpr int PER_WAVE = (NA + NB) / 4;
-  static constexpr int KC = BK * (int)sizeof(TL) / 16;            // chunks per packed row
   static_assert(BYTES_A % 1024 == 0 && BYTES_B % 1024 == 0, "k-step blocks are whole kilobytes");
   static_assert((NA + NB) % 4 == 0, "copies divide evenly over the four waves");
-  static_assert(KC == 2 || KC == 4 || KC == 8, "packed row of 2, 4 or 8 chunks");
-  static_assert(S >= 3 && (S - 2) * PER_WAVE < 64, "ring depth");
+  static_assert((BK * (int)sizeof(TL)) % 32 == 0, "rows of whole 32-byte sub-rows");
+  static_assert(S >= 3 && S * PER_WAVE < 64, "ring depth");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS capacity");
 };
 
-// MFMA fragments from the swizzled, unpadded LDS rows (layouts as Mma<P>::read).
-template <int P, int KC> struct PackedRead;
+// MFMA fragments from the packed LDS layout (the same fragments as
+// Mma<P>::read gives from padded k-contiguous rows).  F = rows of the block.
+template <int P> struct PackedRead;
 
-template <int KC> struct PackedRead<F32, KC> {
-  __device__ static inline f32x4 read(const float* base, int, int row0, int kk, int BK, int lane) {
+template <> struct PackedRead<F32> {
+  __device__ static inline f32x4 read(const float* base, int F, int row0, int kk, int lane) {
     const int r = row0 + (lane & 31), c = (kk >> 2) + (lane >> 5);
-    return *reinterpret_cast<const f32x4*>(base + r * BK + ((c ^ swz<KC>(r)) << 2));
+    return *reinterpret_cast<const f32x4*>(base + chunk_pos<float>(F, r, c));
   }
 };
-template <int KC> struct PackedRead<F64, KC> {
-  __device__ static inline f64x2 read(const double* base, int, int row0, int kk, int BK, int lane) {
+template <> struct PackedRead<F64> {
+  __device__ static inline f64x2 read(const double* base, int F, int row0, int kk, int lane) {
     const int r = row0 + (lane & 15), c = (kk >> 1) + (lane >> 4);
-    return *reinterpret_cast<const f64x2*>(base + r * BK + ((c ^ swz<KC>(r)) << 1));
+    return *reinterpret_cast<const f64x2*>(base + chunk_pos<double>(F, r, c));
   }
 };
-template <int KC> struct PackedRead<BF16, KC> {
-  __device__ static inline bf16x8 read(const __bf16* base, int, int row0, int kk, int BK, int lane) {
+template <> struct PackedRead<BF16> {
+  __device__ static inline bf16x8 read(const __bf16* base, int F, int row0, int kk, int lane) {
     const int r = row0 + (lane & 31), c = (kk >> 3) + (lane >> 5);
-    return *reinterpret_cast<const bf16x8*>(base + r * BK + ((c ^ swz<KC>(r)) << 3));
+    return *reinterpret_cast<const bf16x8*>(base + chunk_pos<__bf16>(F, r, c));
   }
 };
-template <int KC> struct PackedRead<BF16X3, KC> {
-  __device__ static inline Bf16x3Frag read(const __bf16* base, int plane_stride, int row0, int kk, int BK,
-                                           int lane) {
+template <> struct PackedRead<BF16X3> {
+  // planes are F * BK elements apart; BK is implied by the caller's F and
+  // the plane stride argument
+  __device__ static inline Bf16x3Frag read3(const __bf16* base, int F, int plane_stride, int row0, int kk,
+                                            int lane) {
     const int r = row0 + (lane & 31), c = (kk >> 3) + (lane >> 5);
-    const int idx = r * BK + ((c ^ swz<KC>(r)) << 3);
+    const int idx = chunk_pos<__bf16>(F, r, c);
     Bf16x3Frag f;
 #pragma unroll
     for (int q = 0; q < 3; ++q) f.p[q] = *reinterpret_cast<const bf16x8*>(base + q * plane_stride + idx);
     return f;
   }
 };
-template <int KC> struct PackedRead<FP8, KC> {
-  __device__ static inline i32x8 read(const uint8_t* base, int, int row0, int kk, int BK, int lane) {
+template <> struct PackedRead<FP8> {
+  __device__ static inline i32x8 read(const uint8_t* base, int F, int row0, int kk, int lane) {
     const int r = row0 + (lane & 15), c = (kk >> 4) + 2 * (lane >> 4);
-    const uint8_t* row = base + r * BK;
-    const i32x4 lo = *reinterpret_cast<const i32x4*>(row + ((c ^ swz<KC>(r)) << 4));
-    const i32x4 hi = *reinterpret_cast<const i32x4*>(row + (((c + 1) ^ swz<KC>(r)) << 4));
+    const i32x4 lo = *reinterpret_cast<const i32x4*>(base + chunk_pos<uint8_t>(F, r, c));
+    const i32x4 hi = *reinterpret_cast<const i32x4*>(base + chunk_pos<uint8_t>(F, r, c + 1));
     i32x8 f = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     return f;
   }
 };
+
+template <int P, int F, int BK>
+__device__ inline typename Mma<P>::Frag packed_frag(const typename PrecTraits<P>::TL* base, int row0, int kk,
+                                                    int lane) {
+  if constexpr (P == BF16X3) return PackedRead<P>::read3(base, F, F * BK, row0, kk, lane);
+  else return PackedRead<P>::read(base, F, row0, kk, lane);
+}
 
 // This wave's share of the copies of one k-step: copy i (i = w, w + 4, ...)
 // moves kilobyte i of the A block (i < NA) or kilobyte i - NA of the B block.
@@ -130,18 +148,50 @@ __device__ inline void packed_copy(const unsigned char* gA, const unsigned char*
   }
 }
 
+// Fragments of one k-step for this wave (all k sub-steps of the step).
+template <int P, int BM, int BN, int BK, int WM, int WN>
+struct PackedFrags {
+  using TS = TileShape<P, BM, BN, BK, WM, WN>;
+  static constexpr int NKK = BK / PrecTraits<P>::KF;
+  typename Mma<P>::Frag a[NKK][TS::SM], b[NKK][TS::SN];
+
+  __device__ inline void read(const unsigned char* stage, int wm, int wn, int lane) {
+    using TL = typename PrecTraits<P>::TL;
+    using SH = PackedShape<P, BM, BN, BK, 3>;
+    const TL* As = reinterpret_cast<const TL*>(stage);
+    const TL* Bs = reinterpret_cast<const TL*>(stage + SH::BYTES_A);
+#pragma unroll
+    for (int q = 0; q < NKK; ++q) {
+#pragma unroll
+      for (int j = 0; j < TS::SM; ++j)
+        a[q][j] = packed_frag<P, BM, BK>(As, wm * TS::WTM + j * PrecTraits<P>::SUB, q * PrecTraits<P>::KF, lane);
+#pragma unroll
+      for (int i = 0; i < TS::SN; ++i)
+        b[q][i] = packed_frag<P, BN, BK>(Bs, wn * TS::WTN + i * PrecTraits<P>::SUB, q * PrecTraits<P>::KF, lane);
+    }
+  }
+
+  __device__ inline void mma(typename Mma<P>::Acc (&acc)[TS::SN][TS::SM]) const {
+#pragma unroll
+    for (int q = 0; q < NKK; ++q)
+#pragma unroll
+      for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+        for (int j = 0; j < TS::SM; ++j) Mma<P>::mma(acc[i][j], b[q][i], a[q][j]);
+  }
+};
+
 // PERSIST == false: one C tile per workgroup.  PERSIST == true: workgroup
-// blockIdx.x walks tiles blockIdx.x, + gridDim.x, ...; the copies run on
-// across tiles, so the next tile's first k-steps arrive during the epilogue.
+// blockIdx.x walks tiles blockIdx.x, + gridDim.x, ...; the copies and the
+// fragment reads run on across tiles, so the next tile's first k-steps are
+// in flight during the epilogue.
 template <int P, int BM, int BN, int BK, int WM, int WN, int S, bool PERSIST>
 __global__ void __launch_bounds__(256)
 gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
-  using TR = PrecTraits<P>;
-  using TL = typename TR::TL;
   using SH = PackedShape<P, BM, BN, BK, S>;
   using TS = TileShape<P, BM, BN, BK, WM, WN>;
   using MM = Mma<P>;
-  using RD = PackedRead<P, SH::KC>;
+  using FR = PackedFrags<P, BM, BN, BK, WM, WN>;
   extern __shared__ __align__(1024) unsigned char smem_raw[];
 
   const int nblocks = p.tilesM * p.tilesN;
@@ -149,7 +199,7 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   if (v >= nblocks) return;
   const int t = threadIdx.x;
   const int lane = t & 63;
-  const int w = t >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = w % WM, wn = w / WM;
   const int KT = p.kTiles;
   const unsigned char* Ab = reinterpret_cast<const unsigned char*>(p.A);
@@ -157,21 +207,25 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
 
   int tm, tn;
   tile_coords(xcd_chunked(v, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
-
-  // Copy cursor: the next k-step to fetch.  Past the last k-step of the
-  // last tile it stays on that k-step (a harmless re-read), so every step
-  // issues the same number of copies and the vmcnt arithmetic holds.
-  int cv = v, ck = 0, ctm = tm, ctn = tn;
+  // Copy cursor: the next k-step to fetch.  Past the workgroup's last k-step
+  // it stays there (a harmless re-read), so every step issues the same
+  // number of copies and the vmcnt arithmetic holds.
+  int cv = v, ck = 0;
+  const unsigned char* cA = Ab + (size_t)tm * KT * SH::BYTES_A;
+  const unsigned char* cB = Bb + (size_t)tn * KT * SH::BYTES_B;
   auto issue = [&](int slot) {
-    const unsigned char* gA = Ab + ((size_t)ctm * KT + ck) * SH::BYTES_A;
-    const unsigned char* gB = Bb + ((size_t)ctn * KT + ck) * SH::BYTES_B;
-    packed_copy<P, BM, BN, BK, S>(gA, gB, smem_raw + slot * SH::STAGE_BYTES, w, lane);
+    packed_copy<P, BM, BN, BK, S>(cA, cB, smem_raw + slot * SH::STAGE_BYTES, w, lane);
     if (ck + 1 < KT) {
       ++ck;
+      cA += SH::BYTES_A;
+      cB += SH::BYTES_B;
     } else if (PERSIST && cv + (int)gridDim.x < nblocks) {
       cv += gridDim.x;
       ck = 0;
+      int ctm, ctn;
       tile_coords(xcd_chunked(cv, nblocks), p.tilesM, p.tilesN, p.groupM, ctm, ctn);
+      cA = Ab + (size_t)ctm * KT * SH::BYTES_A;
+      cB = Bb + (size_t)ctn * KT * SH::BYTES_B;
     }
   };
 
@@ -182,49 +236,55 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     descale = (a / 448.f) * (b / 448.f);
   }
 
+  typename MM::Acc acc[TS::SN][TS::SM];
 #pragma unroll
-  for (int i = 0; i < S - 1; ++i) issue(i);
-  int slot = 0;
-  bool fresh = false;  // first k-step after an epilogue (its stores count in vmcnt)
+  for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+    for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
 
+#pragma unroll
+  for (int i = 0; i < S; ++i) issue(i);
+  wait_vm_lgkm0<(S - 1) * SH::PER_WAVE>();
+  block_barrier();
+  FR f0, f1;
+  f0.read(smem_raw, wm, wn, lane);
+
+  int slot = 0;   // stage of the current step
+  bool fresh = false;
+  // one k-step: MFMAs from `cur`, fragments of the next step into `nxt`
+  auto step = [&](FR& cur, FR& nxt) {
+    if (PERSIST && fresh) wait_vm_lgkm0<0>();  // the epilogue's accesses count in vmcnt too
+    else wait_vm_lgkm0<(S - 2) * SH::PER_WAVE>();
+    fresh = false;
+    block_barrier();
+    issue(slot);
+    const int nslot = (slot + 1 == S) ? 0 : slot + 1;
+    nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
+    cur.mma(acc);
+    slot = nslot;
+  };
   while (true) {
-    typename MM::Acc acc[TS::SN][TS::SM];
-#pragma unroll
-    for (int i = 0; i < TS::SN; ++i)
-#pragma unroll
-      for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
-
-    for (int k = 0; k < KT; ++k) {
-      if (PERSIST && fresh) wait_vmcnt<0>();
-      else wait_vmcnt<(S - 2) * SH::PER_WAVE>();
-      fresh = false;
-      block_barrier();
-      issue(slot == 0 ? S - 1 : slot - 1);
-      const TL* As = reinterpret_cast<const TL*>(smem_raw + slot * SH::STAGE_BYTES);
-      const TL* Bs = reinterpret_cast<const TL*>(smem_raw + slot * SH::STAGE_BYTES + SH::BYTES_A);
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += TR::KF) {
-        typename MM::Frag af[TS::SM], bf[TS::SN];
-#pragma unroll
-        for (int j = 0; j < TS::SM; ++j) af[j] = RD::read(As, BM * BK, wm * TS::WTM + j * TR::SUB, kk, BK, lane);
-#pragma unroll
-        for (int i = 0; i < TS::SN; ++i) bf[i] = RD::read(Bs, BN * BK, wn * TS::WTN + i * TR::SUB, kk, BK, lane);
-#pragma unroll
-        for (int i = 0; i < TS::SN; ++i)
-#pragma unroll
-          for (int j = 0; j < TS::SM; ++j) MM::mma(acc[i][j], bf[i], af[j]);
-      }
-      slot = (slot + 1 == S) ? 0 : slot + 1;
+    for (int k = 0; k + 1 < KT; k += 2) {
+      step(f0, f1);
+      step(f1, f0);
+    }
+    if (KT & 1) {
+      step(f0, f1);
+      f0 = f1;  // the next tile's first fragments
     }
     store_tile<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale);
     if (!PERSIST) break;
     v += gridDim.x;
     if (v >= nblocks) break;
     tile_coords(xcd_chunked(v, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
+#pragma unroll
+    for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+      for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
     fresh = true;
   }
   // No copy may still be writing this workgroup's LDS when it retires.
-  wait_vmcnt<0>();
+  wait_vm_lgkm0<0>();
 }
 
 }  // namespace tccg
