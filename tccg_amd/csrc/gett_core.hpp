@@ -310,9 +310,15 @@ template <> struct Mma<F32> {
   __device__ static inline Frag read(const float* lds, int, int row0, int kk, int LDK, int lane) {
     return *reinterpret_cast<const f32x4*>(lds + (row0 + (lane & 31)) * LDK + kk + 4 * (lane >> 5));
   }
+  // mma() is PARTS dependent MFMAs; mma_part(q) is the q-th of them, so a
+  // caller can interleave the parts of several accumulators.
+  static constexpr int PARTS = 4;
+  __device__ static inline void mma_part(Acc& acc, const Frag& b, const Frag& a, int s) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b[s], a[s], acc, 0, 0, 0);
+  }
   __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b[s], a[s], acc, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) mma_part(acc, b, a, s);
   }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -327,9 +333,13 @@ template <> struct Mma<F64> {
   __device__ static inline Frag read(const double* lds, int, int row0, int kk, int LDK, int lane) {
     return *reinterpret_cast<const f64x2*>(lds + (row0 + (lane & 15)) * LDK + kk + 2 * (lane >> 4));
   }
+  static constexpr int PARTS = 2;
+  __device__ static inline void mma_part(Acc& acc, const Frag& b, const Frag& a, int s) {
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(b[s], a[s], acc, 0, 0, 0);
+  }
   __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
 #pragma unroll
-    for (int s = 0; s < 2; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(b[s], a[s], acc, 0, 0, 0);
+    for (int s = 0; s < 2; ++s) mma_part(acc, b, a, s);
   }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = (lane >> 4) + 4 * r;
@@ -349,13 +359,17 @@ template <> struct Mma<BF16X3> {
     for (int q = 0; q < 3; ++q) f.p[q] = *reinterpret_cast<const bf16x8*>(lds + q * plane_stride + idx);
     return f;
   }
+  // product q of (b part, a part): (0,2) (1,1) (2,0) (0,1) (1,0) (0,0),
+  // smallest terms first
+  static constexpr int PARTS = 6;
+  __device__ static inline void mma_part(Acc& acc, const Frag& b, const Frag& a, int q) {
+    constexpr int bi[6] = {0, 1, 2, 0, 1, 0};
+    constexpr int ai[6] = {2, 1, 0, 1, 0, 0};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[bi[q]], a.p[ai[q]], acc, 0, 0, 0);
+  }
   __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[2], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[2], a.p[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[1], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[1], a.p[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b.p[0], a.p[0], acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) mma_part(acc, b, a, q);
   }
   __device__ static inline void coord(int lane, int r, int& n, int& m) {
     n = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -370,6 +384,8 @@ template <> struct Mma<BF16> {
   __device__ static inline Frag read(const __bf16* lds, int, int row0, int kk, int LDK, int lane) {
     return *reinterpret_cast<const bf16x8*>(lds + (row0 + (lane & 31)) * LDK + kk + 8 * (lane >> 5));
   }
+  static constexpr int PARTS = 1;
+  __device__ static inline void mma_part(Acc& acc, const Frag& b, const Frag& a, int) { mma(acc, b, a); }
   __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a, acc, 0, 0, 0);
   }
@@ -390,6 +406,8 @@ template <> struct Mma<FP8> {
     Frag f = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     return f;
   }
+  static constexpr int PARTS = 1;
+  __device__ static inline void mma_part(Acc& acc, const Frag& b, const Frag& a, int) { mma(acc, b, a); }
   __device__ static inline void mma(Acc& acc, const Frag& b, const Frag& a) {
     // operand formats 0/0 = FP8 e4m3; block scales 127 = 2^0.
     acc = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(b, a, acc, 0, 0, 0, 127, 0, 127);

@@ -52,11 +52,15 @@ __device__ inline void wait_vm_lgkm0() {
   __builtin_amdgcn_s_waitcnt((N & 0xf) | (0x7 << 4) | (0x0 << 8) | ((N >> 4) << 14));
 }
 
-// Workgroup barrier without a memory fence.  The scheduling barriers keep
-// every instruction (MFMAs included) on its side, so the MFMAs of a k-step
-// stay after the barrier that publishes the fragments read for it.
-__device__ inline void block_barrier() {
+// The step boundary: s_waitcnt vmcnt(N) lgkmcnt(0), then a workgroup barrier
+// without a memory fence.  The scheduling barriers keep every instruction
+// (MFMAs included) on its side, so the wait follows all of the previous
+// step's MFMAs and the MFMAs of a k-step stay after the barrier that
+// publishes the fragments read for it.
+template <int N>
+__device__ inline void step_sync() {
   __builtin_amdgcn_sched_barrier(0);
+  wait_vm_lgkm0<N>();
   asm volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
@@ -153,6 +157,9 @@ template <int P, int BM, int BN, int BK, int WM, int WN>
 struct PackedFrags {
   using TS = TileShape<P, BM, BN, BK, WM, WN>;
   static constexpr int NKK = BK / PrecTraits<P>::KF;
+  // LDS reads and MFMA instructions per k-step of one wave
+  static constexpr int NREAD = NKK * (TS::SM + TS::SN) * (P == BF16X3 ? 3 : P == FP8 ? 2 : 1);
+  static constexpr int NMFMA = NKK * TS::SM * TS::SN * Mma<P>::PARTS;
   typename Mma<P>::Frag a[NKK][TS::SM], b[NKK][TS::SN];
 
   __device__ inline void read(const unsigned char* stage, int wm, int wn, int lane) {
@@ -171,13 +178,17 @@ struct PackedFrags {
     }
   }
 
+  // Part-major order: consecutive MFMAs go to different accumulators; each
+  // accumulator still receives its MFMAs in Mma<P>::mma's order.
   __device__ inline void mma(typename Mma<P>::Acc (&acc)[TS::SN][TS::SM]) const {
 #pragma unroll
     for (int q = 0; q < NKK; ++q)
 #pragma unroll
-      for (int i = 0; i < TS::SN; ++i)
+      for (int part = 0; part < Mma<P>::PARTS; ++part)
 #pragma unroll
-        for (int j = 0; j < TS::SM; ++j) Mma<P>::mma(acc[i][j], b[q][i], a[q][j]);
+        for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+          for (int j = 0; j < TS::SM; ++j) Mma<P>::mma_part(acc[i][j], b[q][i], a[q][j], part);
   }
 };
 
@@ -244,8 +255,7 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
 
 #pragma unroll
   for (int i = 0; i < S; ++i) issue(i);
-  wait_vm_lgkm0<(S - 1) * SH::PER_WAVE>();
-  block_barrier();
+  step_sync<(S - 1) * SH::PER_WAVE>();
   FR f0, f1;
   f0.read(smem_raw, wm, wn, lane);
 
@@ -253,15 +263,26 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   bool fresh = false;
   // one k-step: MFMAs from `cur`, fragments of the next step into `nxt`
   auto step = [&](FR& cur, FR& nxt) {
-    if (PERSIST && fresh) wait_vm_lgkm0<0>();  // the epilogue's accesses count in vmcnt too
-    else wait_vm_lgkm0<(S - 2) * SH::PER_WAVE>();
+    if (PERSIST && fresh) step_sync<0>();  // the epilogue's accesses count in vmcnt too
+    else step_sync<(S - 2) * SH::PER_WAVE>();
     fresh = false;
-    block_barrier();
     issue(slot);
     const int nslot = (slot + 1 == S) ? 0 : slot + 1;
     nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
     cur.mma(acc);
     slot = nslot;
+    // Issue order inside the step: the copies, then the next step's LDS
+    // reads one after each of the first MFMAs (their latency hides behind
+    // the rest of the MFMAs), then the remaining MFMAs.
+    constexpr int NR = FR::NREAD < FR::NMFMA ? FR::NREAD : FR::NMFMA;
+    __builtin_amdgcn_sched_group_barrier(0x020, SH::PER_WAVE, 0);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, FR::NREAD - NR, 0);
+    __builtin_amdgcn_sched_group_barrier(0x008, FR::NMFMA - NR, 0);
   };
   while (true) {
     for (int k = 0; k + 1 < KT; k += 2) {
