@@ -84,6 +84,11 @@ typedef struct tccg_gett_plan {
                        /* persistent workgroups walk the tiles, prefetching */
                        /* the next tile's first k-step during the epilogue  */
   const float* amax;   /* TCCG_FP8 only: device float[2] = max|A|, max|B|   */
+  int32_t walk;        /* 0: as given by grid; 2: streaming walk over grid  */
+                       /* workgroups for single-k-step plans (Kp == BK):    */
+                       /* each owns a contiguous n-fastest tile range, B's  */
+                       /* tiles stay in LDS, A's tile is reloaded per       */
+                       /* m-block (FP32, BK == 8, vector load modes)        */
 } tccg_gett_plan;
 
 /* Tile shape (BM x BN x BK) of tile configuration `tile` for precision `prec`. */

@@ -5,10 +5,10 @@ namespace tccg_lib {
 
 #define TCCG_TILE_CASE(ARR, ID)                                                                    \
   case ID:                                                                                        \
-    return launch_modes<P, ARR[ID].BM, ARR[ID].BN, ARR[ID].BK, ARR[ID].WM, ARR[ID].WN>(ma, mb, prm, grid, s);
+    return launch_modes<P, ARR[ID].BM, ARR[ID].BN, ARR[ID].BK, ARR[ID].WM, ARR[ID].WN>(ma, mb, prm, grid, walk, s);
 
 template <>
-int dispatch_tile<tccg::BF16>(int tile, int ma, int mb, const tccg::GettParams<float>& prm, int grid, hipStream_t s) {
+int dispatch_tile<tccg::BF16>(int tile, int ma, int mb, const tccg::GettParams<float>& prm, int grid, int walk, hipStream_t s) {
   constexpr int P = tccg::BF16;
   switch (tile) {
     TCCG_TILE_CASE(kBf1Tiles, 0)

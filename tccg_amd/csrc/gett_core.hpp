@@ -456,12 +456,15 @@ struct TileShape {
 };
 
 // Epilogue of one tile: alpha/beta update of C along its stride-one index.
-// The C offsets of this lane's rows and columns are gathered before any store.
+// The C offsets of this lane's rows and columns are gathered before any
+// store; cmT / cnT point at the offsets of the tile's first m / n position
+// (the global offset tables, or copies of them in LDS).
 template <int P, int BM, int BN, int BK, int WM, int WN>
-__device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& p,
-                                  typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
-                                                             [TileShape<P, BM, BN, BK, WM, WN>::SM],
-                                  int m0, int n0, int wm, int wn, int lane, float descale) {
+__device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::TG>& p,
+                                       typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                                  [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                       const uint32_t* cmT, const uint32_t* cnT, int wm, int wn, int lane,
+                                       float descale) {
   using TR = PrecTraits<P>;
   using TG = typename TR::TG;
   using S = TileShape<P, BM, BN, BK, WM, WN>;
@@ -472,7 +475,7 @@ __device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& 
   for (int j = 0; j < S::SM; ++j) {
     int n_l, m_l;
     MM::coord(lane, 0, n_l, m_l);
-    cm[j] = p.offCm[m0 + wm * S::WTM + j * TR::SUB + m_l];
+    cm[j] = cmT[wm * S::WTM + j * TR::SUB + m_l];
   }
 #pragma unroll
   for (int i = 0; i < S::SN; ++i)
@@ -480,14 +483,38 @@ __device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& 
     for (int r = 0; r < MM::NACC; ++r) {
       int n_l, m_l;
       MM::coord(lane, r, n_l, m_l);
-      cn[i][r] = p.offCn[n0 + wn * S::WTN + i * TR::SUB + n_l];
+      cn[i][r] = cnT[wn * S::WTN + i * TR::SUB + n_l];
     }
+  // Row by row: the SM column blocks of one C row (consecutive m, i.e.
+  // consecutive addresses along C's stride-one index) are stored back to
+  // back.  Common case first: beta == 0 and every position of the wave's
+  // sub-tile holds an element, so the stores need no per-element checks.
+  bool valid = true;
 #pragma unroll
-  for (int j = 0; j < S::SM; ++j) {
+  for (int j = 0; j < S::SM; ++j) valid = valid && cm[j] != kInvalid;
 #pragma unroll
-    for (int i = 0; i < S::SN; ++i) {
+  for (int i = 0; i < S::SN; ++i)
 #pragma unroll
-      for (int r = 0; r < MM::NACC; ++r) {
+    for (int r = 0; r < MM::NACC; ++r) valid = valid && cn[i][r] != kInvalid;
+  if (beta == TG(0) && __all(valid)) {
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+      for (int r = 0; r < MM::NACC; ++r)
+#pragma unroll
+        for (int j = 0; j < S::SM; ++j) {
+          TG val = (TG)acc[i][j][r];
+          if (P == FP8) val = (TG)((float)val * descale);
+          __builtin_nontemporal_store(alpha * val, p.C + (size_t)(cm[j] + cn[i][r]));
+        }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < S::SN; ++i) {
+#pragma unroll
+    for (int r = 0; r < MM::NACC; ++r) {
+#pragma unroll
+      for (int j = 0; j < S::SM; ++j) {
         if (cm[j] != kInvalid && cn[i][r] != kInvalid) {
           TG val = (TG)acc[i][j][r];
           if (P == FP8) val = (TG)((float)val * descale);
@@ -501,6 +528,14 @@ __device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& 
       }
     }
   }
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& p,
+                                  typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                             [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                  int m0, int n0, int wm, int wn, int lane, float descale) {
+  store_tile_from<P, BM, BN, BK, WM, WN>(p, acc, p.offCm + m0, p.offCn + n0, wm, wn, lane, descale);
 }
 
 // One BM x BN tile (tm, tn) of C over the whole k range, computed by the 256
@@ -765,15 +800,114 @@ __device__ inline void gett_walk(const GettParams<typename PrecTraits<P>::TG>& p
   }
 }
 
-// PERSIST == false: one tile per workgroup (gridDim.x == tiles).
-// PERSIST == true: each workgroup walks several tiles with cross-tile
-// prefetch (gridDim.x <= tiles); worth its extra registers for short k ranges.
-template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB, bool PERSIST>
+// Streaming walk for contractions whose whole k range is one k-step (short
+// k, bandwidth-bound: the time is C's write-back).  Workgroup b owns the
+// contiguous tile range [b*T/G, (b+1)*T/G) in n-fastest order.  Every n-tile
+// of B is packed into LDS once, and A's tile is reloaded only when the range
+// enters the next m-block, so inside an m-block the waves issue nothing but
+// LDS reads, MFMAs and C stores: on gfx9 loads and stores share one in-order
+// vmcnt counter, and a load issued behind a tile's worth of C stores could
+// not be consumed before those stores complete.  For the same reason C's
+// offset tables are copied into LDS (all Np n-offsets once, BM m-offsets per
+// m-block).  LDS: A's tile (BM rows), then all Np rows of B, PLANES planes
+// each, then the Np n-offsets and the BM m-offsets of C.
+template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB>
+__device__ inline void gett_stream(const GettParams<typename PrecTraits<P>::TG>& p, typename PrecTraits<P>::TL* smem) {
+  using TR = PrecTraits<P>;
+  using TL = typename TR::TL;
+  using S = TileShape<P, BM, BN, BK, WM, WN>;
+  using MM = Mma<P>;
+  constexpr int LDK = S::LDK;
+
+  const long T = (long)p.tilesM * p.tilesN;
+  const int G = gridDim.x;
+  const int t0 = (int)(blockIdx.x * T / G), t1 = (int)((blockIdx.x + 1) * T / G);
+  if (t0 >= t1) return;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int w = t >> 6;
+  const int wm = w % WM, wn = w / WM;
+  const int Np = p.tilesN * BN;
+  const int planeB = Np * LDK;
+  TL* As = smem;
+  TL* Bs = smem + TR::PLANES * S::PLANE_A;
+  uint32_t* cnL = reinterpret_cast<uint32_t*>(Bs + TR::PLANES * planeB);
+  uint32_t* cmL = cnL + Np;
+
+  float sA = 1.f, sB = 1.f, descale = 1.f;
+  if (P == FP8) {
+    const float a = p.amax[0] > 0.f ? p.amax[0] : 1.f;
+    const float b = p.amax[1] > 0.f ? p.amax[1] : 1.f;
+    sA = 448.f / a;
+    sB = 448.f / b;
+    descale = (a / 448.f) * (b / 448.f);
+  }
+
+  for (int tn = 0; tn < p.tilesN; ++tn) {
+    TileLoader<P, BN, BK, MODEB> lb;
+    lb.init(t, p.offBn, p.ordB, tn * BN, 1);
+    lb.fetch_k(p.offBk, 0, t);
+    lb.load(p.B, t);
+    lb.store(Bs + tn * BN * LDK, planeB, sB, t);
+  }
+  for (int i = t; i < Np; i += 256) cnL[i] = p.offCn[i];
+
+  int cur_tm = -1;
+  for (int tt = t0; tt < t1; ++tt) {
+    const int tm = tt / p.tilesN, tn = tt % p.tilesN;
+    if (tm != cur_tm) {
+      __syncthreads();  // nobody reads the previous A tile any more
+      TileLoader<P, BM, BK, MODEA> la;
+      la.init(t, p.offAm, p.ordA, tm * BM, 1);
+      la.fetch_k(p.offAk, 0, t);
+      la.load(p.A, t);
+      la.store(As, S::PLANE_A, sA, t);
+      for (int i = t; i < BM; i += 256) cmL[i] = p.offCm[tm * BM + i];
+      __syncthreads();
+      cur_tm = tm;
+    }
+    typename MM::Acc acc[S::SN][S::SM];
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+      for (int j = 0; j < S::SM; ++j) acc[i][j] = typename MM::Acc{};
+    const TL* Bt = Bs + tn * BN * LDK;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += TR::KF) {
+      typename MM::Frag af[S::SM], bf[S::SN];
+#pragma unroll
+      for (int j = 0; j < S::SM; ++j) af[j] = MM::read(As, S::PLANE_A, wm * S::WTM + j * TR::SUB, kk, LDK, lane);
+#pragma unroll
+      for (int i = 0; i < S::SN; ++i) bf[i] = MM::read(Bt, planeB, wn * S::WTN + i * TR::SUB, kk, LDK, lane);
+#pragma unroll
+      for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+        for (int j = 0; j < S::SM; ++j) MM::mma(acc[i][j], bf[i], af[j]);
+    }
+    store_tile_from<P, BM, BN, BK, WM, WN>(p, acc, cmL, cnL + tn * BN, wm, wn, lane, descale);
+  }
+}
+
+// Bytes of LDS gett_stream needs for a problem with tilesN n-tiles.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__host__ __device__ constexpr long stream_lds_bytes(int tilesN) {
+  using S = TileShape<P, BM, BN, BK, WM, WN>;
+  return (long)PrecTraits<P>::PLANES * ((long)BM + (long)tilesN * BN) * S::LDK * (long)sizeof(typename PrecTraits<P>::TL) +
+         4L * ((long)tilesN * BN + BM);
+}
+
+// WALK == 0: one tile per workgroup (gridDim.x == tiles).
+// WALK == 1: each workgroup walks several tiles with cross-tile prefetch
+//            (gridDim.x <= tiles); worth its extra registers for short k ranges.
+// WALK == 2: streaming walk of single-k-step contractions (gett_stream).
+template <int P, int BM, int BN, int BK, int WM, int WN, int MODEA, int MODEB, int WALK>
 __global__ void __launch_bounds__(256)
 gett_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   auto* smem = reinterpret_cast<typename PrecTraits<P>::TL*>(smem_raw);
-  if constexpr (PERSIST) {
+  if constexpr (WALK == 2) {
+    gett_stream<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, smem);
+  } else if constexpr (WALK == 1) {
     gett_walk<P, BM, BN, BK, WM, WN, MODEA, MODEB>(p, smem);
   } else {
     const int nblocks = p.tilesM * p.tilesN;

@@ -5,10 +5,10 @@ namespace tccg_lib {
 
 #define TCCG_TILE_CASE(ARR, ID)                                                                    \
   case ID:                                                                                        \
-    return launch_modes<P, ARR[ID].BM, ARR[ID].BN, ARR[ID].BK, ARR[ID].WM, ARR[ID].WN>(ma, mb, prm, grid, s);
+    return launch_modes<P, ARR[ID].BM, ARR[ID].BN, ARR[ID].BK, ARR[ID].WM, ARR[ID].WN>(ma, mb, prm, grid, walk, s);
 
 template <>
-int dispatch_tile<tccg::F32>(int tile, int ma, int mb, const tccg::GettParams<float>& prm, int grid, hipStream_t s) {
+int dispatch_tile<tccg::F32>(int tile, int ma, int mb, const tccg::GettParams<float>& prm, int grid, int walk, hipStream_t s) {
   constexpr int P = tccg::F32;
   switch (tile) {
     TCCG_TILE_CASE(kF32Tiles, 0)
@@ -20,6 +20,9 @@ int dispatch_tile<tccg::F32>(int tile, int ma, int mb, const tccg::GettParams<fl
     TCCG_TILE_CASE(kF32Tiles, 6)
     TCCG_TILE_CASE(kF32Tiles, 7)
     TCCG_TILE_CASE(kF32Tiles, 8)
+    TCCG_TILE_CASE(kF32Tiles, 9)
+    TCCG_TILE_CASE(kF32Tiles, 10)
+    TCCG_TILE_CASE(kF32Tiles, 11)
     default: return fail(1, "invalid tile id");
   }
 }

@@ -18,7 +18,8 @@ struct TileCfg { int BM, BN, BK, WM, WN; };
 // Tile configurations per precision; ids index these arrays.
 constexpr TileCfg kF32Tiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 256, 16, 2, 2},
                                  {64, 64, 32, 2, 2},   {192, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
-                                 {128, 128, 8, 2, 2},  {64, 384, 8, 1, 4},   {128, 256, 8, 2, 2}};
+                                 {128, 128, 8, 2, 2},  {64, 384, 8, 1, 4},   {128, 256, 8, 2, 2},
+                                 {512, 32, 8, 4, 1},   {256, 64, 8, 4, 1},   {1024, 32, 8, 4, 1}};
 constexpr TileCfg kF64Tiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}, {192, 128, 16, 2, 2}};
 constexpr TileCfg kBfTiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
 constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192, 128, 32, 2, 2}, {128, 128, 16, 2, 2}};
@@ -73,44 +74,81 @@ inline const TileCfg* tiles_of(int prec, int* n) {
   }
 }
 
-template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB, bool PERSIST>
+constexpr int kMaxLds = 160 * 1024;
+
+// Workgroups of `kern` (256 threads, `lds` bytes) that fit on the current
+// device at once: a walking grid larger than this would run in rounds.
+inline int resident_limit(const void* kern, int lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess) return 0;
+  return cus * per_cu;
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB, int WALK>
 int launch_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t stream) {
   using S = tccg::TileShape<P, BM, BN, BK, WM, WN>;
-  auto kern = tccg::gett_kernel<P, BM, BN, BK, WM, WN, MA, MB, PERSIST>;
+  auto kern = tccg::gett_kernel<P, BM, BN, BK, WM, WN, MA, MB, WALK>;
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, S::LDS_BYTES);
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       WALK == 2 ? kMaxLds : S::LDS_BYTES);
     if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     attr_set = true;
   }
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), S::LDS_BYTES, stream, prm);
+  // A single k-step never touches the second LDS stage.
+  int lds = prm.kTiles == 1 ? S::LDS_BYTES / 2 : S::LDS_BYTES;
+  if (WALK == 2) lds = (int)tccg::stream_lds_bytes<P, BM, BN, BK, WM, WN>(prm.tilesN);
+  if (WALK != 0) {
+    const int cap = resident_limit(reinterpret_cast<const void*>(kern), lds);
+    if (cap > 0 && grid > cap) grid = cap;
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, stream, prm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
   return 0;
 }
 
-// grid == 0: one tile per workgroup; grid > 0: persistent walk over that many workgroups.
+// Whether the streaming walk is built for this tile: FP32, one 8-deep k-step
+// (the bandwidth-bound short-k contractions), vector load modes.
+template <int P, int BK, int MA, int MB>
+constexpr bool stream_built() {
+  return P == tccg::F32 && BK == 8 && (MA == tccg::FREE4 || MA == tccg::KDIM4) &&
+         (MB == tccg::FREE4 || MB == tccg::KDIM4);
+}
+
+// walk 0: grid == 0 one tile per workgroup, grid > 0 persistent walk over that
+// many workgroups; walk 2: streaming walk over `grid` workgroups.
 template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB>
-int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t stream) {
+int launch_one(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, int walk,
+               hipStream_t stream) {
   const int nblocks = prm.tilesM * prm.tilesN;
+  if (walk == 2) {
+    if constexpr (stream_built<P, BK, MA, MB>()) {
+      return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, 2>(prm, grid, stream);
+    }
+    return fail(1, "the streaming walk is not built for this precision, tile and load modes");
+  }
   // The persistent walk is built for the vector load modes only; plans with a
   // scalar mode run one tile per workgroup.
   constexpr bool kVec = (MA == tccg::FREE4 || MA == tccg::KDIM4) && (MB == tccg::FREE4 || MB == tccg::KDIM4);
   if constexpr (kVec) {
-    if (grid > 0 && grid < nblocks) return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, true>(prm, grid, stream);
+    if (grid > 0 && grid < nblocks) return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, 1>(prm, grid, stream);
   }
-  return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, false>(prm, nblocks, stream);
+  return launch_kernel<P, BM, BN, BK, WM, WN, MA, MB, 0>(prm, nblocks, stream);
 }
 
 template <int P, int BM, int BN, int BK, int WM, int WN>
-int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t s) {
+int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, int walk,
+                 hipStream_t s) {
 #define TCCG_MB(MA_)                                                                       \
   switch (mb) {                                                                            \
-    case 0: return launch_one<P, BM, BN, BK, WM, WN, MA_, 0>(prm, grid, s);                \
-    case 1: return launch_one<P, BM, BN, BK, WM, WN, MA_, 1>(prm, grid, s);                \
-    case 2: return launch_one<P, BM, BN, BK, WM, WN, MA_, 2>(prm, grid, s);                \
-    case 3: return launch_one<P, BM, BN, BK, WM, WN, MA_, 3>(prm, grid, s);                \
+    case 0: return launch_one<P, BM, BN, BK, WM, WN, MA_, 0>(prm, grid, walk, s);          \
+    case 1: return launch_one<P, BM, BN, BK, WM, WN, MA_, 1>(prm, grid, walk, s);          \
+    case 2: return launch_one<P, BM, BN, BK, WM, WN, MA_, 2>(prm, grid, walk, s);          \
+    case 3: return launch_one<P, BM, BN, BK, WM, WN, MA_, 3>(prm, grid, walk, s);          \
     default: return fail(1, "invalid load mode of B");                                     \
   }
   switch (ma) {
@@ -126,7 +164,7 @@ int launch_modes(int ma, int mb, const tccg::GettParams<typename tccg::PrecTrait
 
 template <int P>
 int dispatch_tile(int tile, int ma, int mb, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm,
-                  int grid, hipStream_t s);
+                  int grid, int walk, hipStream_t s);
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
@@ -167,7 +205,16 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
   if (plan->grid < 0) return fail(1, "grid must be >= 0");
-  return dispatch_tile<P>(plan->tile, plan->modeA, plan->modeB, prm, plan->grid, (hipStream_t)stream);
+  if (plan->walk != 0 && plan->walk != 2) return fail(1, "walk must be 0 or 2");
+  if (plan->walk == 2) {
+    if (plan->grid <= 0) return fail(1, "the streaming walk needs grid > 0");
+    if (prm.kTiles != 1) return fail(1, "the streaming walk needs the whole k range in one k-step (Kp == BK)");
+    const long lds = (long)(tc.BM + (long)prm.tilesN * tc.BN) * (tc.BK + tccg::PrecTraits<P>::PAD) *
+                         tccg::PrecTraits<P>::PLANES * (long)sizeof(typename tccg::PrecTraits<P>::TL) +
+                     4L * ((long)prm.tilesN * tc.BN + tc.BM);
+    if (lds > kMaxLds) return fail(1, "the streaming walk needs B's n-tiles and A's tile in LDS (too large)");
+  }
+  return dispatch_tile<P>(plan->tile, plan->modeA, plan->modeB, prm, plan->grid, plan->walk, (hipStream_t)stream);
 }
 
 

@@ -39,7 +39,7 @@ FREE4, FREE1, KDIM4, KDIM1 = 0, 1, 2, 3
 # the tables of csrc/gett_launch.hpp (checked against the library at load time).
 TILES: Dict[int, List[Tuple[int, int, int]]] = {
     F32: [(128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 128, 16), (128, 128, 16),
-          (128, 128, 8), (64, 384, 8), (128, 256, 8)],
+          (128, 128, 8), (64, 384, 8), (128, 256, 8), (512, 32, 8), (256, 64, 8), (1024, 32, 8)],
     F64: [(128, 128, 16), (64, 64, 16), (192, 128, 16)],
     BF16X3: [(128, 128, 32), (64, 64, 32), (192, 128, 32), (128, 128, 16)],
     FP8: [(128, 128, 128), (64, 64, 128), (192, 128, 128)],
@@ -58,7 +58,8 @@ PACK_TK = 32      # k positions per pack block: packed plans pad Kp to a multipl
 _ELEM_BYTES = {F32: 4, F64: 8, BF16X3: 2, FP8: 1, BF16: 2}
 
 # Wave arrangement (WM x WN) of each tile; (2, 2) unless listed.
-WAVES: Dict[Tuple[int, int], Tuple[int, int]] = {(F32, 7): (1, 4)}
+WAVES: Dict[Tuple[int, int], Tuple[int, int]] = {(F32, 7): (1, 4), (F32, 9): (4, 1), (F32, 10): (4, 1),
+                                                 (F32, 11): (4, 1)}
 
 _LDS_PAD = {F32: (4, 4), F64: (2, 8), BF16X3: (8, 2), FP8: (16, 1), BF16: (8, 2)}  # (pad elements, element bytes)
 _PLANES = {F32: 1, F64: 1, BF16X3: 3, FP8: 1, BF16: 1}
@@ -288,6 +289,7 @@ class GettPlan:
     groupM: int = 8
     grid: int = 0          # 0: one workgroup per tile; > 0: persistent walk
     packed: bool = False   # operands converted once into tile-major planes
+    walk: int = 0          # 2: streaming walk over `grid` workgroups (single k-step)
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -318,9 +320,14 @@ class GettPlan:
 
     def describe(self) -> str:
         mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
+        walk = ""
+        if self.walk == 2:
+            walk = " stream grid=%d" % self.grid
+        elif self.grid:
+            walk = " persistent grid=%d" % self.grid
         return ("prec=%s tile=%d (%dx%dx%d)%s%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
                 % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, " packed" if self.packed else "",
-                   (" persistent grid=%d" % self.grid) if self.grid else "", self.lin_m.inner,
+                   walk, self.lin_m.inner,
                    self.lin_n.inner, self.lin_k.inner, mode[self.modeA], mode[self.modeB], self.ntiles(),
                    self.Mp, self.Np, self.Kp))
 
@@ -393,8 +400,22 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
     return plan
 
 
+def stream_lds_bytes(prec: int, BM: int, BN: int, BK: int, Np: int) -> int:
+    """LDS of the streaming walk: A's tile plus every n-tile of B (csrc/gett_core.hpp)."""
+    pad, eb = _LDS_PAD[prec]
+    return _PLANES[prec] * (BM + Np) * (BK + pad) * eb + 4 * (Np + BM)
+
+
+def stream_ok(plan: "GettPlan") -> bool:
+    """The streaming walk is built for FP32 single-k-step (BK == 8) plans with
+    vector load modes whose LDS footprint fits."""
+    return (plan.prec == F32 and plan.BK == 8 and plan.Kp == plan.BK and not plan.packed
+            and plan.modeA in (FREE4, KDIM4) and plan.modeB in (FREE4, KDIM4)
+            and stream_lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK, plan.Np) <= MI355X["lds_bytes_per_cu"])
+
+
 def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] = None,
-               groupM: int = 8, persistent: bool = False) -> GettPlan:
+               groupM: int = 8, persistent: bool = False, stream: bool = False) -> GettPlan:
     BM, BN, BK = TILES[prec][tile]
     m_l, n_l, k_l = _orders(pb)
     m_e, n_e, k_e = _ext(pb, m_l), _ext(pb, n_l), _ext(pb, k_l)
@@ -432,7 +453,11 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
     modeB = mode(pb.opB, n_l, offBn, ordB, BN, offBk)
     plan = GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
                     ordA, ordB, modeA, modeB, groupM)
-    if persistent and plan.modeA in (FREE4, KDIM4) and plan.modeB in (FREE4, KDIM4):
+    if stream and stream_ok(plan):
+        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // stream_lds_bytes(prec, BM, BN, BK, plan.Np)))
+        plan.walk = 2
+        plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
+    elif persistent and plan.modeA in (FREE4, KDIM4) and plan.modeB in (FREE4, KDIM4):
         per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // lds_bytes(prec, BM, BN, BK)))
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
         if plan.grid >= plan.ntiles():
@@ -485,7 +510,9 @@ def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
     esz = 8 if plan.prec == F64 else 4
     t_store = esz * plan.BM * plan.BN / (0.7 * hw["hbm_bytes_per_s"] / slots)
     t_fixed = 2.0e-6 + t_store
-    if plan.grid:
+    if plan.walk == 2:
+        t_fixed = 0.1e-6 + t_store * 0.5
+    elif plan.grid:
         t_fixed = 0.4e-6 + t_store * 0.6
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
     t = rounds * (t_mfma + t_fixed) + t_pack
@@ -503,11 +530,15 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
     for tile in range(len(TILES[prec])):
         for blocked in ((False, True) if conflict else (False,)):
             plans.append(build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM))
-            # short k ranges: also the persistent walk with cross-tile prefetch
+            # short k ranges: also the persistent walk with cross-tile prefetch,
+            # and for a single k-step the streaming walk
             if pb.K <= 256:
                 pp = build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM, persistent=True)
                 if pp.grid:
                     plans.append(pp)
+                ps = build_plan(pb, prec, tile, m_blocked=blocked, groupM=groupM, stream=True)
+                if ps.walk == 2:
+                    plans.append(ps)
     for tile in range(len(PACKED_TILES[prec])):
         plans.append(build_packed_plan(pb, prec, tile, groupM=groupM))
         if pb.K <= 256:

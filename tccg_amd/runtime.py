@@ -32,7 +32,7 @@ class _CPlan(ctypes.Structure):
         ("Mp", ctypes.c_int32), ("Np", ctypes.c_int32), ("Kp", ctypes.c_int32),
         ("tile", ctypes.c_int32), ("modeA", ctypes.c_int32), ("modeB", ctypes.c_int32),
         ("groupM", ctypes.c_int32), ("grid", ctypes.c_int32),
-        ("amax", ctypes.c_void_p),
+        ("amax", ctypes.c_void_p), ("walk", ctypes.c_int32),
     ]
 
 
@@ -143,6 +143,7 @@ class DevicePlan:
         c.Mp, c.Np, c.Kp = plan.Mp, plan.Np, plan.Kp
         c.tile, c.modeA, c.modeB, c.groupM = plan.tile, plan.modeA, plan.modeB, plan.groupM
         c.grid = plan.grid
+        c.walk = plan.walk
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
         self.workspace = None

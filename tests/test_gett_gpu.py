@@ -121,3 +121,34 @@ def test_golden_fixtures(path, prec):
     c(tA, tB, tC, prec=prec)
     torch.cuda.synchronize()
     assert rel_frobenius(tC.cpu().numpy(), z["C_out"]) < 1e-6
+
+
+# single-k-step contractions (K <= 8): the streaming walk, every tile that has it
+STREAM_CASES = [
+    (["a", "b", "c"], ["a", "k", "b"], ["k", "c"], dict(a=96, b=40, c=72, k=8)),   # bandwidth-bound pattern
+    (["a", "b", "c"], ["a", "k", "b"], ["k", "c"], dict(a=36, b=17, c=65, k=4)),   # ragged tiles, k padded
+    (["a", "b"], ["a", "k"], ["b", "k"], dict(a=200, b=96, k=8)),                 # B along k
+]
+
+
+@pytest.mark.parametrize("ci", range(len(STREAM_CASES)))
+@pytest.mark.parametrize("grid", [None, 3])
+def test_stream_walk(ci, grid):
+    from tccg_amd.plan import stream_ok
+    c_l, a_l, b_l, sizes = STREAM_CASES[ci]
+    c = Contraction.from_labels(c_l, a_l, b_l, sizes)
+    ran = 0
+    for tile, (BM, BN, BK) in enumerate(TILES[F32]):
+        if BK != 8:
+            continue
+        plan = build_plan(c.problem, F32, tile, stream=True)
+        if plan.walk != 2:
+            assert not stream_ok(plan)
+            continue
+        if grid is not None:
+            plan.grid = min(grid, plan.ntiles())
+        print("stream", ci, plan.describe(), flush=True)
+        check(c, F32, seed=ci, plan=plan)
+        check(c, F32, seed=ci, plan=plan, alpha=2.0, beta=-0.5)
+        ran += 1
+    assert ran > 0
