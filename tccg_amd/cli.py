@@ -29,7 +29,8 @@ import time
 from pathlib import Path
 
 from .frontend import TccgSyntaxError, contraction_str, fuse_indices, read_tccg_file
-from .plan import BF16X3, F32, F64, FP8, PREC_NAMES, UnsupportedContraction, candidate_plans, build_plan, estimate_seconds
+from .plan import (BF16X3, F32, F64, FP8, PREC_NAMES, UnsupportedContraction, candidate_plans, build_plan,
+                   estimate_seconds, plan_from_kind, plan_kind)
 
 OKGREEN = "\033[92m"
 FAIL = "\033[91m"
@@ -89,13 +90,16 @@ def settings_str(args, prec: int) -> str:
 def candidate_name(plan) -> str:
     """Reference-style candidate name (tccg/gett.py:578-579); decodable by the
     reference's decodeName (tccg/gett.py:561-576)."""
-    from .plan import TILES
+    from .plan import WAVES, plan_kind
     pb = plan.problem
     blocked = 0 if plan.lin_m.is_plain() else 1
-    wtm, wtn = plan.BM // 2, plan.BN // 2
+    # var: bit 0 blocked m linearisation, higher bits the plan variant
+    var = blocked | (plan_kind(plan) << 1)
+    WM, WN = (2, 2) if plan.packed else WAVES.get((plan.prec, plan.tile), (2, 2))
+    wtm, wtn = plan.BM // WM, plan.BN // WN
     sub = 16 if plan.prec in (F64, FP8) else 32
     return "gett_var%d_mc%d_nc%d_kc%d_mc1%d_nc1%d_mr%d_nr%d_%s_%s_%s" % (
-        blocked, plan.BM, plan.BN, plan.BK, wtm, wtn, sub, sub, pb.opA.index_str(), pb.opB.index_str(),
+        var, plan.BM, plan.BN, plan.BK, wtm, wtn, sub, sub, pb.opA.index_str(), pb.opB.index_str(),
         pb.C.index_str())
 
 
@@ -169,8 +173,8 @@ def main(argv=None) -> int:
         device = "gfx950"
         hit = None if args.ignoreDatabase else db.fastest(contraction, device, PREC_NAMES[prec], contraction.beta)
         if hit is not None:
-            name, tile, blocked, gf = hit
-            chosen = build_plan(pb, prec, tile, m_blocked=blocked)
+            name, tile, blocked, kind, gf = hit
+            chosen = plan_from_kind(pb, prec, tile, blocked, kind)
             best_gflops = gf
             print(OKGREEN + "[database] reusing %s (%.2f GFLOPS)" % (name, gf) + ENDC)
         else:
@@ -233,7 +237,7 @@ def _time_candidates(args, contraction, prec, plans, db, device):
         gf = contraction.flops() / t / 1e9
         est = contraction.flops() / estimate_seconds(p, contraction.beta) / 1e9
         print("%s: %.2f GFLOPS // estimated: %.2f GFLOPS" % (name, gf, est))
-        db.insert_gett(mid, name, p.tile, not p.lin_m.is_plain(), gf, est)
+        db.insert_gett(mid, name, p.tile, not p.lin_m.is_plain(), gf, est, plan_kind(p))
         if gf > best[0]:
             best = (gf, p)
     if failed:

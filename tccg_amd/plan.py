@@ -465,6 +465,28 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
     return plan
 
 
+# Plan variants as stored by the tuning database: bit 0 packed operands,
+# bit 1 persistent walk, bit 2 streaming walk.
+KIND_PACKED, KIND_PERSISTENT, KIND_STREAM = 1, 2, 4
+
+
+def plan_kind(plan: GettPlan) -> int:
+    kind = KIND_PACKED if plan.packed else 0
+    if plan.walk == 2:
+        kind |= KIND_STREAM
+    elif plan.grid:
+        kind |= KIND_PERSISTENT
+    return kind
+
+
+def plan_from_kind(pb: GettProblem, prec: int, tile: int, m_blocked: bool, kind: int, groupM: int = 8) -> GettPlan:
+    """The plan a tuning-database entry (tile, m_blocked, kind) describes."""
+    if kind & KIND_PACKED:
+        return build_packed_plan(pb, prec, tile, groupM=groupM, persistent=bool(kind & KIND_PERSISTENT))
+    return build_plan(pb, prec, tile, m_blocked=m_blocked, groupM=groupM,
+                      persistent=bool(kind & KIND_PERSISTENT), stream=bool(kind & KIND_STREAM))
+
+
 # ---------------------------------------------------------------------------
 # Performance model (gfx950 analogue of Gett.estimateGFLOPS, tccg/gett.py:590-687)
 # ---------------------------------------------------------------------------

@@ -29,7 +29,8 @@ CREATE TABLE IF NOT EXISTS gett (
   gett_id INTEGER PRIMARY KEY AUTOINCREMENT,
   measurement_id INTEGER NOT NULL REFERENCES measurements(measurement_id),
   name TEXT NOT NULL, tile INTEGER NOT NULL, m_blocked INTEGER NOT NULL,
-  gflops REAL NOT NULL, estimated_gflops REAL NOT NULL
+  gflops REAL NOT NULL, estimated_gflops REAL NOT NULL,
+  kind INTEGER NOT NULL DEFAULT 0
 );
 """
 
@@ -47,6 +48,10 @@ class TuningDB:
         self.path.parent.mkdir(parents=True, exist_ok=True)
         self.con = sqlite3.connect(str(self.path))
         self.con.executescript(SCHEMA)
+        cols = [r[1] for r in self.con.execute("PRAGMA table_info(gett)").fetchall()]
+        if "kind" not in cols:   # databases written before plan variants were recorded
+            self.con.execute("ALTER TABLE gett ADD COLUMN kind INTEGER NOT NULL DEFAULT 0")
+            self.con.commit()
 
     def close(self) -> None:
         self.con.close()
@@ -69,19 +74,21 @@ class TuningDB:
         self.con.commit()
         return int(cur.lastrowid)
 
-    def insert_gett(self, mid: int, name: str, tile: int, m_blocked: bool, gflops: float, estimated: float) -> None:
-        self.con.execute("INSERT INTO gett(measurement_id, name, tile, m_blocked, gflops, estimated_gflops) "
-                         "VALUES (?,?,?,?,?,?)", (mid, name, int(tile), int(bool(m_blocked)), float(gflops),
-                                                  float(estimated)))
+    def insert_gett(self, mid: int, name: str, tile: int, m_blocked: bool, gflops: float, estimated: float,
+                    kind: int = 0) -> None:
+        self.con.execute("INSERT INTO gett(measurement_id, name, tile, m_blocked, gflops, estimated_gflops, kind) "
+                         "VALUES (?,?,?,?,?,?,?)", (mid, name, int(tile), int(bool(m_blocked)), float(gflops),
+                                                    float(estimated), int(kind)))
         self.con.commit()
 
-    def fastest(self, contraction, device: str, prec: str, beta: float) -> Optional[Tuple[str, int, bool, float]]:
-        """(name, tile, m_blocked, gflops) of the fastest stored candidate."""
+    def fastest(self, contraction, device: str, prec: str, beta: float) -> Optional[Tuple[str, int, bool, int, float]]:
+        """(name, tile, m_blocked, kind, gflops) of the fastest stored candidate
+        (kind: tccg_amd.plan.KIND_* bits)."""
         mid = self.measurement_id(contraction, device, prec, beta)
         if mid < 0:
             return None
-        row = self.con.execute("SELECT name, tile, m_blocked, gflops FROM gett WHERE measurement_id=? "
+        row = self.con.execute("SELECT name, tile, m_blocked, kind, gflops FROM gett WHERE measurement_id=? "
                                "ORDER BY gflops DESC LIMIT 1", (mid,)).fetchone()
         if not row:
             return None
-        return str(row[0]), int(row[1]), bool(row[2]), float(row[3])
+        return str(row[0]), int(row[1]), bool(row[2]), int(row[3]), float(row[4])

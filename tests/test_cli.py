@@ -115,3 +115,57 @@ def test_generated_tables_are_exported_device_symbols(tmp_path):
     for key in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB"):
         sym = "tccg_%s_%s" % (name, key)
         assert re.search(r"^\s*\.globl\s+%s\s*$" % sym, text, flags=re.M), sym
+
+
+def test_plan_kind_round_trip():
+    """Every candidate plan is rebuilt exactly from its tuning-database entry."""
+    from tccg_amd import Contraction
+    from tccg_amd.plan import BF16X3, F32, candidate_plans, plan_from_kind, plan_kind
+    from tccg_amd.workloads import get
+    for wl in ("abc-akb-kc", "abcdef-dgfa-gceb"):
+        c = Contraction.from_tccg(get(wl))
+        for prec in (F32, BF16X3):
+            kinds = set()
+            for p in candidate_plans(c.problem, prec):
+                k = plan_kind(p)
+                kinds.add(k)
+                q = plan_from_kind(c.problem, prec, p.tile, not p.lin_m.is_plain(), k)
+                assert q.describe() == p.describe()
+                assert all((a == b).all() for a, b in ((p.offAm, q.offAm), (p.offAk, q.offAk), (p.offCn, q.offCn)))
+                assert candidate_name(q) == candidate_name(p)
+            assert any(k & 1 for k in kinds)   # packed plans are candidates
+
+
+def test_tuning_db_kind_and_migration(tmp_path):
+    import sqlite3
+    from tccg_amd import Contraction
+    from tccg_amd.tuning_db import TuningDB
+    path = tmp_path / "old.db"
+    con = sqlite3.connect(str(path))   # schema without the kind column
+    con.executescript("""
+CREATE TABLE measurements (measurement_id INTEGER PRIMARY KEY AUTOINCREMENT, tensorA TEXT NOT NULL,
+  tensorB TEXT NOT NULL, tensorC TEXT NOT NULL, device TEXT NOT NULL, prec TEXT NOT NULL,
+  beta_nonzero INTEGER NOT NULL, created REAL NOT NULL, UNIQUE(tensorA, tensorB, tensorC, device, prec, beta_nonzero));
+CREATE TABLE gett (gett_id INTEGER PRIMARY KEY AUTOINCREMENT, measurement_id INTEGER NOT NULL,
+  name TEXT NOT NULL, tile INTEGER NOT NULL, m_blocked INTEGER NOT NULL, gflops REAL NOT NULL,
+  estimated_gflops REAL NOT NULL);
+""")
+    con.close()
+    db = TuningDB(path)
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    mid = db.measurement_id(c, "gfx950", "bf16x3", 0.0, create=True)
+    db.insert_gett(mid, "a", 1, False, 10.0, 5.0, kind=0)
+    db.insert_gett(mid, "b", 2, True, 30.0, 5.0, kind=3)
+    assert db.fastest(c, "gfx950", "bf16x3", 0.0) == ("b", 2, True, 3, 30.0)
+    db.close()
+
+
+def test_emit_packed_source():
+    from tccg_amd import Contraction
+    from tccg_amd.plan import BF16X3, build_packed_plan
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    plan = build_packed_plan(c.problem, BF16X3, 0)
+    src = codegen.emit_source(plan, WORKLOADS["example"], "gett")
+    assert "gett_packed_kernel<tccg::BF16X3, 128, 128, 16" in src
+    assert src.count("tccg::pack_kernel<tccg::BF16X3") == 2
+    assert "hipMalloc(&t.ws, %d)" % plan.workspace_bytes() in src

@@ -57,3 +57,25 @@ def test_cli_testing_on_gpu(tmp_path):
                            cwd=str(tmp_path), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                            timeout=600)
     assert proc2.returncode == 0 and "[database] reusing" in proc2.stdout
+
+
+@pytest.mark.parametrize("wl,prec,tol", [("abcd-aebf-dfce", 2, 1e-5), ("abcdef-dgfa-gceb", 3, 8e-2)])
+def test_generated_packed_library(wl, prec, tol):
+    """Generated libraries of packed plans (pack kernels + DMA-ring GETT kernel,
+    workspace allocated by the library on its first call)."""
+    import torch
+    lib_path = codegen.GENERATED_DIR / ("lib%s_packed.so" % codegen.workload_library_name(wl, prec))
+    assert lib_path.exists(), "generated library missing: run __graft_entry__.build()"
+    g = GeneratedGett(lib_path)
+    assert g.info["prec"] == prec
+    c = Contraction.from_tccg(WORKLOADS[wl])
+    A, B, C = inputs(c, seed=5, dist="normal")
+    dev = torch.device("cuda", 0)
+    tA, tB, tC = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+    ref = contract_ref([i.label for i in c.C0.indices], [i.label for i in c.A0.indices],
+                       [i.label for i in c.B0.indices], A, B)
+    for _ in range(2):   # the second call reuses the workspace
+        tC.zero_()
+        g(tA, tB, tC, 1.0, 0.0)
+        torch.cuda.synchronize()
+        assert rel_frobenius(tC.cpu().numpy(), ref) < tol
