@@ -89,6 +89,14 @@ typedef struct tccg_gett_plan {
                        /* each owns a contiguous n-fastest tile range, B's  */
                        /* tiles stay in LDS, A's tile is reloaded per       */
                        /* m-block (FP32, BK == 8, vector load modes)        */
+  int32_t crun_m;      /* packed plans, one tile per workgroup: when > 0,   */
+  int32_t crun_n;      /* C's stride-one index spans crun_m consecutive m   */
+                       /* positions and C's next index crun_n consecutive n */
+                       /* positions (offCm[m + a] = offCm[m] + a,           */
+                       /* offCn[n + b] = offCn[n] + b * crun_m), tiles hold */
+                       /* whole blocks and C is 16-byte aligned: the tile   */
+                       /* is written as crun_m * crun_n element runs.       */
+                       /* 0 for the plain epilogue.                         */
 } tccg_gett_plan;
 
 /* Tile shape (BM x BN x BK) of tile configuration `tile` for precision `prec`. */

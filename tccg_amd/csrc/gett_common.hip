@@ -82,8 +82,10 @@ int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream) {
   if (!x || !out || n <= 0) return fail(1, "amax: bad arguments");
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(amax_zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out));
+  // one atomic per block: few blocks (each walks a strided range) keep the
+  // atomics on the single result word from serialising
   int64_t blocks = (n + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 256) blocks = 256;
   hipLaunchKernelGGL(amax_kernel, dim3((unsigned)blocks), dim3(256), 0, s, x, n, reinterpret_cast<unsigned*>(out));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("amax launch: ") + hipGetErrorString(e));

@@ -79,6 +79,11 @@ struct GettParams {
   int groupM;
   TG alpha, beta;
   const float* amax;  // FP8 only: device float[2] = max|A|, max|B|
+  // Packed kernel epilogue: when crunM > 0, C's stride-one index spans crunM
+  // consecutive m positions and its next index crunN consecutive n
+  // positions, every tile holds whole crunM x crunN blocks, and each block
+  // is crunM * crunN contiguous elements of C (staged through LDS).
+  int crunM, crunN;
 };
 
 template <int P> struct PrecTraits;

@@ -35,7 +35,8 @@ constexpr TileCfg kBfPackedTiles[] = {{128, 128, 16, 2, 2}, {128, 128, 32, 2, 2}
                                       {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2},   {192, 192, 16, 2, 2}};
 constexpr TileCfg kBf1PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 32, 2, 2}, {128, 256, 32, 2, 2},
                                        {64, 64, 32, 2, 2},   {192, 192, 32, 2, 2}};
-constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2}};
+constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2},
+                                      {192, 192, 128, 2, 2}};
 
 constexpr int packed_count(int prec) {
   return prec == tccg::F32 ? (int)(sizeof(kF32PackedTiles) / sizeof(TileCfg))
@@ -201,6 +202,8 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   prm.groupM = plan->groupM > 0 ? plan->groupM : 1;
   prm.alpha = alpha; prm.beta = beta;
   prm.amax = plan->amax;
+  prm.crunM = 0;
+  prm.crunN = 0;
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
@@ -271,6 +274,15 @@ int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTra
     constexpr TileCfg c = packed_tile(P, I);
     if (tile == I) {
       const int nblocks = prm.tilesM * prm.tilesN;
+      if (prm.crunM > 0) {
+        using TG = typename tccg::PrecTraits<P>::TG;
+        constexpr int VEC = 16 / (int)sizeof(TG);
+        constexpr int rows_max = tccg::PackedShape<P, c.BM, c.BN, c.BK, kPackedStages>::LDS_BYTES /
+                                 ((c.BM + VEC) * (int)sizeof(TG));
+        if (prm.crunM % VEC || c.BM % prm.crunM || c.BN % prm.crunN || rows_max < prm.crunN)
+          return fail(1, "crun_m/crun_n do not fit the tile (whole runs, 16-byte vectors, LDS rows)");
+        if (grid > 0 && grid < nblocks) return fail(1, "the run epilogue is built for one tile per workgroup");
+      }
       if (grid > 0 && grid < nblocks) return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, true>(prm, grid, s);
       return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, false>(prm, nblocks, s);
     }
@@ -365,6 +377,11 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   prm.groupM = plan->groupM > 0 ? plan->groupM : 1;
   prm.alpha = alpha; prm.beta = beta;
   prm.amax = plan->amax;
+  prm.crunM = plan->crun_m;
+  prm.crunN = plan->crun_n;
+  if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
+    return fail(1, "crun_m and crun_n must both be positive or both zero");
+  if (plan->crun_m > 0 && !aligned16(C)) return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);

@@ -192,6 +192,74 @@ struct PackedFrags {
   }
 };
 
+// Epilogue through LDS for C layouts whose contiguous runs span both m and
+// n (C's stride-one index short and its next index in the n set): the tile
+// is staged as [n][m] rows of LDS in passes of whole crunN-row groups, then
+// each crunM x crunN block is written as one contiguous run with 16-byte
+// stores.  All copies into the LDS ring must have landed before the call.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__device__ inline void store_tile_staged(const GettParams<typename PrecTraits<P>::TG>& p,
+                                         typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                                    [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                         int m0, int n0, int wm, int wn, int lane, float descale,
+                                         unsigned char* lds_raw, int lds_bytes) {
+  using TR = PrecTraits<P>;
+  using TG = typename TR::TG;
+  using S = TileShape<P, BM, BN, BK, WM, WN>;
+  using MM = Mma<P>;
+  typedef TG vec_t __attribute__((ext_vector_type(16 / sizeof(TG))));
+  constexpr int VEC = 16 / (int)sizeof(TG);
+  constexpr int LD = BM + VEC;  // staged row length (elements), 16-byte multiple
+  TG* L = reinterpret_cast<TG*>(lds_raw);
+  const int E0 = p.crunM, E1 = p.crunN;
+  const int rows = (lds_bytes / (LD * (int)sizeof(TG))) / E1 * E1;
+  const int runvec = E0 * E1 / VEC, mruns = BM / E0;
+  const int t = threadIdx.x;
+  const TG alpha = p.alpha, beta = p.beta;
+  // this thread's first vector inside a run: element e = a + b * E0
+  const int a0 = (t * VEC) % E0, b0 = (t * VEC) / E0;
+  for (int p0 = 0; p0 < BN; p0 += rows) {
+    const int p1 = min(BN, p0 + rows);
+#pragma unroll
+    for (int i = 0; i < S::SN; ++i)
+#pragma unroll
+      for (int r = 0; r < MM::NACC; ++r) {
+        int n_l, m_l;
+        MM::coord(lane, r, n_l, m_l);
+        const int n = wn * S::WTN + i * TR::SUB + n_l;
+        if (n < p0 || n >= p1) continue;
+#pragma unroll
+        for (int j = 0; j < S::SM; ++j) {
+          TG val = (TG)acc[i][j][r];
+          if (P == FP8) val = (TG)((float)val * descale);
+          L[(n - p0) * LD + wm * S::WTM + j * TR::SUB + m_l] = val;
+        }
+      }
+    __syncthreads();
+    const int nruns = (p1 - p0) / E1;
+    for (int run = 0; run < nruns * mruns; ++run) {
+      const int mo = run % mruns, no = run / mruns;
+      TG* dst = p.C + (size_t)(p.offCm[m0 + mo * E0] + p.offCn[n0 + p0 + no * E1]);
+      const TG* src = L + (size_t)(no * E1) * LD + mo * E0;
+      int a = a0, b = b0;
+      for (int e4 = t; e4 < runvec; e4 += 256) {
+        if (e4 != t) {
+          a = (e4 * VEC) % E0;
+          b = (e4 * VEC) / E0;
+        }
+        vec_t v = *reinterpret_cast<const vec_t*>(src + b * LD + a);
+        vec_t* g = reinterpret_cast<vec_t*>(dst + e4 * VEC);
+        if (beta == TG(0)) {
+          __builtin_nontemporal_store(alpha * v, g);
+        } else {
+          *g = alpha * v + beta * (*g);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // PERSIST == false: one C tile per workgroup.  PERSIST == true: workgroup
 // blockIdx.x walks tiles blockIdx.x, + gridDim.x, ...; the copies and the
 // fragment reads run on across tiles, so the next tile's first k-steps are
@@ -253,36 +321,64 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
 #pragma unroll
     for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
 
+  // Fragment double-buffering (the next step's fragments read during this
+  // step's MFMAs) except for FP8, whose fragments are twice as large: there
+  // the second register set would halve the waves per SIMD.
+  constexpr bool DB = P != FP8;
+  // copies in flight ahead of the MFMAs: S steps with double-buffering (the
+  // stage of step s is free once its fragments are in registers), else S - 1
+  constexpr int AHEAD = DB ? S : S - 1;
 #pragma unroll
-  for (int i = 0; i < S; ++i) issue(i);
-  step_sync<(S - 1) * SH::PER_WAVE>();
+  for (int i = 0; i < AHEAD; ++i) issue(i);
   FR f0, f1;
-  f0.read(smem_raw, wm, wn, lane);
+  if constexpr (DB) {
+    step_sync<(S - 1) * SH::PER_WAVE>();
+    f0.read(smem_raw, wm, wn, lane);
+  }
 
   int slot = 0;   // stage of the current step
   bool fresh = false;
-  // one k-step: MFMAs from `cur`, fragments of the next step into `nxt`
+  // one k-step: MFMAs from `cur`; with double-buffering the fragments of the
+  // next step go into `nxt`, else `cur` is read from this step's stage first
   auto step = [&](FR& cur, FR& nxt) {
-    if (PERSIST && fresh) step_sync<0>();  // the epilogue's accesses count in vmcnt too
+    // The copies waited for: with double-buffering the next step's (whose
+    // fragments this step reads), else this step's.  After an epilogue its
+    // memory operations also count in vmcnt.  Counts complete in issue
+    // order, and behind the awaited copies the wave always issued the later
+    // copies plus the epilogue's C-offset loads (SM + SN * NACC,
+    // unconditional; its stores may be fewer, whole-padding stores are
+    // skipped).  Waiting until at most that many operations are outstanding
+    // guarantees the copies without waiting for the C stores to drain (a
+    // count below the exact one only waits more; vmcnt stops at 63).
+    constexpr int kAfter = (S - 2) * SH::PER_WAVE + TS::SM + TS::SN * MM::NACC;
+    constexpr int kFresh = kAfter < 63 ? kAfter : 63;
+    if (PERSIST && fresh) step_sync<kFresh>();
     else step_sync<(S - 2) * SH::PER_WAVE>();
     fresh = false;
-    issue(slot);
     const int nslot = (slot + 1 == S) ? 0 : slot + 1;
-    nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
+    if constexpr (DB) {
+      issue(slot);
+      nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
+    } else {
+      issue(slot == 0 ? S - 1 : slot - 1);
+      cur.read(smem_raw + slot * SH::STAGE_BYTES, wm, wn, lane);
+    }
     cur.mma(acc);
     slot = nslot;
     // Issue order inside the step: the copies, then the next step's LDS
     // reads one after each of the first MFMAs (their latency hides behind
     // the rest of the MFMAs), then the remaining MFMAs.
-    constexpr int NR = FR::NREAD < FR::NMFMA ? FR::NREAD : FR::NMFMA;
-    __builtin_amdgcn_sched_group_barrier(0x020, SH::PER_WAVE, 0);
+    if constexpr (DB) {
+      constexpr int NR = FR::NREAD < FR::NMFMA ? FR::NREAD : FR::NMFMA;
+      __builtin_amdgcn_sched_group_barrier(0x020, SH::PER_WAVE, 0);
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      for (int i = 0; i < NR; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x100, FR::NREAD - NR, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, FR::NMFMA - NR, 0);
     }
-    __builtin_amdgcn_sched_group_barrier(0x100, FR::NREAD - NR, 0);
-    __builtin_amdgcn_sched_group_barrier(0x008, FR::NMFMA - NR, 0);
   };
   while (true) {
     for (int k = 0; k + 1 < KT; k += 2) {
@@ -291,9 +387,16 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     }
     if (KT & 1) {
       step(f0, f1);
-      f0 = f1;  // the next tile's first fragments
+      if constexpr (DB) f0 = f1;  // the next tile's first fragments
     }
-    store_tile<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale);
+    if (!PERSIST && p.crunM > 0) {
+      wait_vm_lgkm0<0>();  // the ring is reused for staging: every copy has landed
+      __syncthreads();
+      store_tile_staged<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale, smem_raw,
+                                               SH::LDS_BYTES);
+    } else {
+      store_tile<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale);
+    }
     if (!PERSIST) break;
     v += gridDim.x;
     if (v >= nblocks) break;

@@ -51,7 +51,7 @@ PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
     F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16)],
     F64: [(128, 128, 16), (64, 64, 16)],
     BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 192, 16)],
-    FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128)],
+    FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128), (192, 192, 128)],
     BF16: [(128, 128, 32), (256, 128, 32), (128, 256, 32), (64, 64, 32), (192, 192, 32)],
 }
 PACK_TK = 32      # k positions per pack block: packed plans pad Kp to a multiple
@@ -290,6 +290,7 @@ class GettPlan:
     grid: int = 0          # 0: one workgroup per tile; > 0: persistent walk
     packed: bool = False   # operands converted once into tile-major planes
     walk: int = 0          # 2: streaming walk over `grid` workgroups (single k-step)
+    crun: Tuple[int, int] = (0, 0)  # packed: (E0, E1) blocks of C written as contiguous runs
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -325,6 +326,8 @@ class GettPlan:
             walk = " stream grid=%d" % self.grid
         elif self.grid:
             walk = " persistent grid=%d" % self.grid
+        if self.crun[0]:
+            walk += " runs=%dx%d" % self.crun
         return ("prec=%s tile=%d (%dx%dx%d)%s%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
                 % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, " packed" if self.packed else "",
                    walk, self.lin_m.inner,
@@ -392,6 +395,10 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
     ident_n = np.arange(BN, dtype=np.int32)
     plan = GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
                     ident_m, ident_n, modeA, modeB, groupM, packed=True)
+    if not persistent:
+        esz = 8 if prec == F64 else 4
+        ring = 3 * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec]   # LDS of the packed kernel
+        plan.crun = c_runs(offCm, offCn, BM, BN, esz, ring // ((BM + 16 // esz) * esz))
     if persistent:
         per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // lds_bytes(prec, BM, BN, BK)))
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
@@ -412,6 +419,41 @@ def stream_ok(plan: "GettPlan") -> bool:
     return (plan.prec == F32 and plan.BK == 8 and plan.Kp == plan.BK and not plan.packed
             and plan.modeA in (FREE4, KDIM4) and plan.modeB in (FREE4, KDIM4)
             and stream_lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK, plan.Np) <= MI355X["lds_bytes_per_cu"])
+
+
+def _leading_run(x: np.ndarray, step: int) -> int:
+    d = np.diff(x)
+    bad = np.nonzero(d != step)[0]
+    return int(bad[0]) + 1 if bad.size else int(x.size)
+
+
+def c_runs(offCm: np.ndarray, offCn: np.ndarray, BM: int, BN: int, esz: int, rows_max: int) -> Tuple[int, int]:
+    """(E0, E1) when C's contiguous runs span both index sets: offCm is made of
+    runs of E0 consecutive offsets (E0 < BM divides BM, a whole number of
+    16-byte vectors, aligned), offCn of runs of E1 offsets E0 apart (E1
+    divides BN, at most rows_max), with no invalid entries.  Then each E0 x E1
+    block of a tile is E0 * E1 contiguous elements of C and the packed kernel
+    writes it as one run.  The largest such blocks are chosen; (0, 0) if none."""
+    cm = offCm.astype(np.int64)
+    cn = offCn.astype(np.int64)
+    if np.any(cm == np.int64(INVALID)) or np.any(cn == np.int64(INVALID)) or cm.size < 2 or cn.size < 2:
+        return 0, 0
+    vec = 16 // esz
+    Rm = _leading_run(cm, 1)
+    E0 = max([d for d in _divisors(Rm) if BM % d == 0 and d % vec == 0 and d < BM], default=0)
+    if not E0 or cm.size % E0:
+        return 0, 0
+    r = cm.reshape(-1, E0)
+    if not (np.all(r - r[:, :1] == np.arange(E0)) and np.all(r[:, 0] % vec == 0)):
+        return 0, 0
+    Rn = _leading_run(cn, E0)
+    E1 = max([d for d in _divisors(Rn) if BN % d == 0 and 2 <= d <= rows_max], default=0)
+    if not E1 or cn.size % E1:
+        return 0, 0
+    q = cn.reshape(-1, E1)
+    if not np.all(q - q[:, :1] == E0 * np.arange(E1)):
+        return 0, 0
+    return E0, E1
 
 
 def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] = None,
@@ -501,6 +543,10 @@ MI355X = {
 
 
 def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
+    return _estimate(plan, beta)[0]
+
+
+def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     """Estimate of one launch, the gfx950 analogue of Gett.estimateGFLOPS
     (tccg/gett.py:590-687): per tile, MFMA time of the padded work plus a fixed
     cost (gather latency and C write-back at the slot's share of HBM
@@ -540,7 +586,7 @@ def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
     t = rounds * (t_mfma + t_fixed) + t_pack
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
     t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
-    return max(t, t_hbm)
+    return max(t, t_hbm), t
 
 
 def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPlan]:
@@ -567,7 +613,8 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
             pp = build_packed_plan(pb, prec, tile, groupM=groupM, persistent=True)
             if pp.grid:
                 plans.append(pp)
-    plans.sort(key=lambda p: estimate_seconds(p))
+    # ties at the HBM floor are broken by the time the tile work alone would take
+    plans.sort(key=lambda p: _estimate(p))
     return plans
 
 

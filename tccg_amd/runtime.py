@@ -33,6 +33,7 @@ class _CPlan(ctypes.Structure):
         ("tile", ctypes.c_int32), ("modeA", ctypes.c_int32), ("modeB", ctypes.c_int32),
         ("groupM", ctypes.c_int32), ("grid", ctypes.c_int32),
         ("amax", ctypes.c_void_p), ("walk", ctypes.c_int32),
+        ("crun_m", ctypes.c_int32), ("crun_n", ctypes.c_int32),
     ]
 
 
@@ -144,6 +145,8 @@ class DevicePlan:
         c.tile, c.modeA, c.modeB, c.groupM = plan.tile, plan.modeA, plan.modeB, plan.groupM
         c.grid = plan.grid
         c.walk = plan.walk
+        # the run epilogue is built for one tile per workgroup
+        c.crun_m, c.crun_n = plan.crun if not plan.grid else (0, 0)
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
         self.workspace = None

@@ -115,3 +115,24 @@ def test_packed_workspace_too_small_is_rejected():
                                      None)
     assert rc != 0
     assert "workspace" in lib.tccg_last_error().decode()
+
+
+@pytest.mark.parametrize("prec", [F32, BF16X3, FP8])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (2.4, 1.1)])
+def test_packed_run_epilogue(prec, alpha, beta):
+    """C's contiguous runs span m and n (the precision contraction's layout at
+    extents 12): the 192 x 192 tile writes whole runs staged through LDS."""
+    text = """C[a,b,c,d,e,f] = A[d,g,f,a] * B[g,c,e,b]
+a = 12
+b = 12
+c = 12
+d = 12
+e = 12
+f = 12
+g = 40
+"""
+    c = Contraction.from_tccg(text)
+    tile = PACKED_TILES[prec].index(next(t for t in PACKED_TILES[prec] if t[:2] == (192, 192)))
+    plan = build_packed_plan(c.problem, prec, tile)
+    assert plan.crun[0] == 12 and plan.crun[1] > 12, plan.crun
+    check(c, prec, seed=13, alpha=alpha, beta=beta, plan=plan, tol=(8e-2 if prec == FP8 else None))
