@@ -40,6 +40,8 @@ def parse_args():
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (PMC calibration)")
+    ap.add_argument("--candidate", type=int, default=None,
+                    help="skip the autotuner: run the K-th model-ranked candidate plan (profiling a chosen plan)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -68,24 +70,43 @@ def fp64_reference(c, A, B):
 
 
 def timed_steps(dp, A, B, C, steps: int, warmup: int):
-    """(wall seconds of `steps` launches bracketed by synchronize, average
-    per-launch HIP-event duration in seconds)."""
+    """(wall seconds of `steps` back-to-back calls bracketed by synchronize,
+    average per-call GPU time in seconds from HIP events recorded on the
+    launch stream before the first and after the last call: the GPU stays
+    busy, so host launch latency is not counted)."""
     import torch
     s = torch.cuda.current_stream()
     for _ in range(warmup):
         dp.run(A, B, C)
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    for e0, e1 in evs:
-        e0.record(s)
+    e0.record(s)
+    for _ in range(steps):
         dp.run(A, B, C)
-        e1.record(s)
+    e1.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern = sum(e0.elapsed_time(e1) for e0, e1 in evs) / steps * 1e-3
-    return wall, kern
+    return wall, e0.elapsed_time(e1) / steps * 1e-3
+
+
+def kernel_only(dp, C, steps: int):
+    """Average HIP-event duration (seconds) of the GETT kernel alone: for a
+    packed plan the kernel over the operands the last run() packed, for an
+    unpacked plan the single launch of run() itself (None then)."""
+    import torch
+    if not dp.plan.packed:
+        return None
+    s = torch.cuda.current_stream()
+    dp.run_prepacked(C)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        dp.run_prepacked(C)
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps * 1e-3
 
 
 def cold_min(dp, A, B, C, reps: int = 5) -> float:
@@ -125,19 +146,28 @@ def run_form(name, wl, prec, args, log):
     from tccg_amd.plan import PREC_BY_NAME
     p = PREC_BY_NAME[prec]
     t0 = time.perf_counter()
-    ms = autotune(c, p, max_implementations=args.max_implementations, reps=3, inputs=(A, B, C),
-                  verbose=args.verbose)
+    if args.candidate is not None:
+        from tccg_amd.plan import candidate_plans
+        best = candidate_plans(c.problem, p)[args.candidate]
+        ms = [best]
+    else:
+        ms = autotune(c, p, max_implementations=args.max_implementations, reps=3, inputs=(A, B, C),
+                      verbose=args.verbose)
+        best = ms[0].plan
     tune_s = time.perf_counter() - t0
-    best = ms[0].plan
     dp = DevicePlan(best)
-    wall, kern = timed_steps(dp, A, B, C, args.steps, args.warmup)
+    wall, call = timed_steps(dp, A, B, C, args.steps, args.warmup)
+    kern = kernel_only(dp, C, args.steps)
+    if kern is None:
+        kern = call
     cold = cold_min(dp, A, B, C)
     flops = c.flops()
     bytes_ = A.element_size() * (A.numel() + B.numel() + C.numel())
     res = {
         "workload": str(c), "prec": prec, "M": c.problem.M, "N": c.problem.N, "K": c.problem.K,
         "plan": best.describe(), "tuned_candidates": len(ms), "tune_seconds": round(tune_s, 2),
-        "wall_ms_per_step": wall / args.steps * 1e3, "kernel_ms": kern * 1e3, "cold_min_ms": cold * 1e3,
+        "wall_ms_per_step": wall / args.steps * 1e3, "call_ms": call * 1e3, "kernel_ms": kern * 1e3,
+        "cold_min_ms": cold * 1e3,
         "gflops": flops / (wall / args.steps) / 1e9, "kernel_gflops": flops / kern / 1e9,
         "kernel_gbs": bytes_ / kern / 1e9, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
     }
@@ -152,8 +182,8 @@ def run_form(name, wl, prec, args, log):
         ref = fp64_reference(c, A, B)
         res["rel_err_vs_fp64_einsum"] = rel_err(C, ref)
         del ref
-    log("%-7s %-18s %9.1f GFLOP/s (kernel %.3f ms, cold %.3f ms) %s" % (
-        prec, wl, res["kernel_gflops"], res["kernel_ms"], res["cold_min_ms"], best.describe()))
+    log("%-7s %-18s %9.1f GFLOP/s (kernel %.3f ms, call %.3f ms, cold %.3f ms) %s" % (
+        prec, wl, res["kernel_gflops"], res["kernel_ms"], res["call_ms"], res["cold_min_ms"], best.describe()))
     del A, B, C
     torch.cuda.empty_cache()
     return res
@@ -249,6 +279,21 @@ def run_grad(args, log):
     return out
 
 
+def profiled_traffic(key: str, plan: str):
+    """HBM bytes per launch of the dominant kernel from profiles/traffic.json
+    (calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE of the same workload and
+    plan); None when no profile of this exact plan is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    if not entry or entry.get("plan") != plan:
+        return None
+    return entry.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(args, log):
     """The oracle's AVX2 GETT restatement on the host cores, on a bounded sample
     of the headline workload (whole contraction repeated for about
@@ -332,7 +377,8 @@ def main():
                    "alpha": 1.0, "beta": 0.0, "plan": head["plan"]},
         "roofline": {"bound": "mfma" if head is not results.get("bw") else "hbm",
                      "achieved": round(achieved_tf, 2), "peak": PEAK_TFLOPS[head["prec"]], "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / PEAK_TFLOPS[head["prec"]], 4), "traffic": None},
+                     "frac": round(achieved_tf / PEAK_TFLOPS[head["prec"]], 4),
+                     "traffic": profiled_traffic("%s %s" % (head["workload"], head["prec"]), head["plan"])},
         "cpu_baseline": cpu,
         "forms": results,
         "details": details,

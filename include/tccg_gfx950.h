@@ -149,6 +149,12 @@ int tccg_gett_packed_bf16(const float* A, const float* B, float* C, float alpha,
                           const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
 int tccg_gett_packed_fp8(const float* A, const float* B, float* C, float alpha, float beta,
                          const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+/* The GETT kernel alone over operands that a previous tccg_gett_packed_<prec>
+ * call with the same plan and workspace has packed (A and B unchanged since):
+ * for callers that reuse packed operands across calls, and for timing the
+ * kernel apart from the packing.  alpha/beta are converted to the plan's type. */
+int tccg_gett_prepacked(int prec, void* C, double alpha, double beta, const tccg_gett_plan* plan,
+                        void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Batched GETT: many independent FP32 contractions in ONE persistent launch.

@@ -77,6 +77,30 @@ size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan) {
   return tccg_lib::packed_workspace_bytes(prec, plan, nullptr);
 }
 
+int tccg_gett_prepacked(int prec, void* C, double alpha, double beta, const tccg_gett_plan* plan, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+  using namespace tccg_lib;
+  switch (prec) {
+    case tccg::F32:
+      return run_gett_packed<tccg::F32>(nullptr, nullptr, (float*)C, (float)alpha, (float)beta, plan, workspace,
+                                        workspace_bytes, stream, false);
+    case tccg::F64:
+      return run_gett_packed<tccg::F64>(nullptr, nullptr, (double*)C, alpha, beta, plan, workspace, workspace_bytes,
+                                        stream, false);
+    case tccg::BF16X3:
+      return run_gett_packed<tccg::BF16X3>(nullptr, nullptr, (float*)C, (float)alpha, (float)beta, plan, workspace,
+                                           workspace_bytes, stream, false);
+    case tccg::BF16:
+      return run_gett_packed<tccg::BF16>(nullptr, nullptr, (float*)C, (float)alpha, (float)beta, plan, workspace,
+                                         workspace_bytes, stream, false);
+    case tccg::FP8:
+      return run_gett_packed<tccg::FP8>(nullptr, nullptr, (float*)C, (float)alpha, (float)beta, plan, workspace,
+                                        workspace_bytes, stream, false);
+    default:
+      return fail(1, "unknown precision");
+  }
+}
+
 int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream) {
   using namespace tccg_lib;
   if (!x || !out || n <= 0) return fail(1, "amax: bad arguments");

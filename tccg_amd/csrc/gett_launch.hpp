@@ -327,15 +327,18 @@ int pack_operands(int tile, const typename tccg::PrecTraits<P>::TG* A, const typ
   }
 }
 
+// pack == true: pack A and B into the workspace, then the GETT kernel;
+// pack == false: the GETT kernel alone over operands a previous call with
+// the same plan and workspace has packed (A and B are not read).
 template <int P>
 int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
                     typename tccg::PrecTraits<P>::TG* C, typename tccg::PrecTraits<P>::TG alpha,
                     typename tccg::PrecTraits<P>::TG beta, const tccg_gett_plan* plan, void* workspace,
-                    size_t workspace_bytes, void* stream) {
+                    size_t workspace_bytes, void* stream, bool pack = true) {
   using TG = typename tccg::PrecTraits<P>::TG;
   using TL = typename tccg::PrecTraits<P>::TL;
   if (!plan) return fail(1, "plan is NULL");
-  if (!A || !B || !C) return fail(1, "tensor pointer is NULL");
+  if ((pack && (!A || !B)) || !C) return fail(1, "tensor pointer is NULL");
   if (!workspace) return fail(1, "workspace is NULL");
   if (!aligned16(workspace)) return fail(1, "workspace must be 16-byte aligned");
   if (plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0) return fail(1, "table lengths must be positive");
@@ -361,8 +364,10 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   TL* wA = reinterpret_cast<TL*>(workspace);
   TL* wB = reinterpret_cast<TL*>(reinterpret_cast<unsigned char*>(workspace) + offB);
   hipStream_t s = (hipStream_t)stream;
-  int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
-  if (rc) return rc;
+  if (pack) {
+    int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
+    if (rc) return rc;
+  }
   tccg::GettParams<TG> prm;
   prm.A = reinterpret_cast<const TG*>(wA);
   prm.B = reinterpret_cast<const TG*>(wB);

@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_
                     "tccg_batch_num_variants", "tccg_batch_variant", "tccg_gett_batch_f32",
                     "tccg_packed_workspace_bytes", "tccg_packed_tile_shape", "tccg_num_packed_tiles",
                     "tccg_gett_packed_f32", "tccg_gett_packed_f64", "tccg_gett_packed_bf16x3",
-                    "tccg_gett_packed_bf16", "tccg_gett_packed_fp8"]
+                    "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked"]
 
 _PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
               BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
@@ -80,6 +80,8 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
             T = f64 if prec == F64 else f32
             fn.argtypes = [vp, vp, vp, T, T, ctypes.POINTER(_CPlan), vp, ctypes.c_size_t, vp]
             fn.restype = ctypes.c_int
+        lib.tccg_gett_prepacked.argtypes = [ctypes.c_int, vp, f64, f64, ctypes.POINTER(_CPlan), vp, ctypes.c_size_t, vp]
+        lib.tccg_gett_prepacked.restype = ctypes.c_int
         lib.tccg_packed_workspace_bytes.argtypes = [ctypes.c_int, ctypes.POINTER(_CPlan)]
         lib.tccg_packed_workspace_bytes.restype = ctypes.c_size_t
         lib.tccg_packed_tile_shape.argtypes = lib.tccg_tile_shape.argtypes
@@ -197,6 +199,20 @@ class DevicePlan:
             fn = {F32: lib.tccg_gett_f32, F64: lib.tccg_gett_f64, BF16X3: lib.tccg_gett_bf16x3,
                   FP8: lib.tccg_gett_fp8, BF16: lib.tccg_gett_bf16}[prec]
             rc = fn(opA.data_ptr(), opB.data_ptr(), C.data_ptr(), alpha, beta, ctypes.byref(self.cplan), sptr)
+        if rc != 0:
+            raise GettRuntimeError("GETT launch failed: %s" % last_error())
+
+
+    def run_prepacked(self, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        """Packed plans only: the GETT kernel alone over the operands the last
+        run() packed into this plan's workspace (A and B unchanged since)."""
+        torch = _torch()
+        if not self.plan.packed:
+            raise GettRuntimeError("run_prepacked() needs a packed plan")
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        rc = load_library().tccg_gett_prepacked(self.plan.prec, C.data_ptr(), alpha, beta, ctypes.byref(self.cplan),
+                                                self.workspace.data_ptr(), self.workspace.numel(),
+                                                ctypes.c_void_p(s.cuda_stream))
         if rc != 0:
             raise GettRuntimeError("GETT launch failed: %s" % last_error())
 

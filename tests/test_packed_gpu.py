@@ -136,3 +136,21 @@ g = 40
     plan = build_packed_plan(c.problem, prec, tile)
     assert plan.crun[0] == 12 and plan.crun[1] > 12, plan.crun
     check(c, prec, seed=13, alpha=alpha, beta=beta, plan=plan, tol=(8e-2 if prec == FP8 else None))
+
+
+@pytest.mark.parametrize("prec", [F32, BF16X3, FP8])
+def test_prepacked_reuses_packed_operands(prec):
+    """tccg_gett_prepacked over the operands the previous call packed gives
+    the same bits as the full call."""
+    import torch
+    c = Contraction.from_tccg(scaled("abcd-aebf-dfce", 0.25))
+    dp = c.device_plan(plan=build_packed_plan(c.problem, prec, 0))
+    A, B, C = inputs(c, 2, "normal")
+    dev = torch.device("cuda", 0)
+    tA, tB, tC = (torch.from_numpy(np.ascontiguousarray(x)).to(dev) for x in (A, B, C))
+    dp.run(tA, tB, tC)
+    first = tC.clone()
+    tC.zero_()
+    dp.run_prepacked(tC)
+    torch.cuda.synchronize()
+    assert torch.equal(first, tC)
