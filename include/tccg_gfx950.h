@@ -88,7 +88,14 @@ typedef struct tccg_gett_plan {
                        /* workgroups for single-k-step plans (Kp == BK):    */
                        /* each owns a contiguous n-fastest tile range, B's  */
                        /* tiles stay in LDS, A's tile is reloaded per       */
-                       /* m-block (FP32, BK == 8, vector load modes)        */
+                       /* m-block (FP32, BK == 8, vector load modes);       */
+                       /* 3: vector micro-kernel for short k ranges (FP32, */
+                       /* FP64): `tile` indexes tccg_vector_tile_shape(),   */
+                       /* Kp == BK, grid 0; modeA TCCG_FREE4 asserts that   */
+                       /* every aligned group of four m positions is four   */
+                       /* consecutive, 4-aligned offsets in both A and C    */
+                       /* (or invalid as a whole), any other mode reads and */
+                       /* writes one element at a time                      */
   int32_t crun_m;      /* packed plans, one tile per workgroup: when > 0,   */
   int32_t crun_n;      /* C's stride-one index spans crun_m consecutive m   */
                        /* positions and C's next index crun_n consecutive n */
@@ -104,6 +111,13 @@ int tccg_tile_shape(int prec, int tile, int* BM, int* BN, int* BK);
 
 /* Number of tile configurations defined for `prec` (ids 0..n-1). */
 int tccg_num_tiles(int prec);
+
+/* Tiles of the vector micro-kernel (walk 3): the reference's vector FMA
+ * micro-kernel (tccg/gett.py:145-176) for contractions whose k range is at
+ * most BK (16), where the MFMA tiles gain nothing and C's write-back is the
+ * whole cost.  FP32 and FP64 only; other precisions have 0 tiles. */
+int tccg_num_vector_tiles(int prec);
+int tccg_vector_tile_shape(int prec, int tile, int* BM, int* BN, int* BK);
 
 /* Plan-driven GETT.  stream is a hipStream_t (NULL = null stream). */
 int tccg_gett_f32(const float* A, const float* B, float* C, float alpha, float beta,

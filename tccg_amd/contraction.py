@@ -15,7 +15,7 @@ from typing import Dict, Optional, Sequence, Tuple
 
 from .frontend import contraction_str, fuse_indices, parse_tccg
 from .ir import Index, Tensor
-from .plan import F32, F64, PREC_BY_NAME, PREC_NAMES, GettPlan, GettProblem, build_plan, default_plan
+from .plan import F32, F64, PREC_BY_NAME, PREC_NAMES, GettPlan, GettProblem, build_plan, candidate_plans, default_plan
 
 
 def _prec_id(prec) -> int:
@@ -80,7 +80,8 @@ class Contraction:
         if tile is None and m_blocked is None:
             return default_plan(self.problem, p)
         if tile is None:
-            tile = default_plan(self.problem, p).tile
+            # `tile` indexes the gather kernel's table: the model's best plan of that kernel
+            tile = next(q.tile for q in candidate_plans(self.problem, p) if not q.packed and q.walk != 3)
         return build_plan(self.problem, p, tile, m_blocked=m_blocked)
 
     def device_plan(self, prec="f32", tile: Optional[int] = None, m_blocked: Optional[bool] = None,

@@ -73,6 +73,23 @@ int tccg_packed_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
   return 0;
 }
 
+int tccg_num_vector_tiles(int prec) {
+  int n = 0;
+  tccg_lib::vector_tiles_of(prec, &n);
+  return n;
+}
+
+int tccg_vector_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
+  int n = 0;
+  const tccg_lib::TileCfg* t = tccg_lib::vector_tiles_of(prec, &n);
+  if (!t) return tccg_lib::fail(1, "no vector micro-kernel for this precision");
+  if (tile < 0 || tile >= n) return tccg_lib::fail(1, "invalid vector tile id");
+  if (BM) *BM = t[tile].BM;
+  if (BN) *BN = t[tile].BN;
+  if (BK) *BK = t[tile].BK;
+  return 0;
+}
+
 size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan) {
   return tccg_lib::packed_workspace_bytes(prec, plan, nullptr);
 }

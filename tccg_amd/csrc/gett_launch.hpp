@@ -7,6 +7,7 @@
 
 #include "gett_core.hpp"
 #include "gett_pack.hpp"
+#include "gett_vector.hpp"
 #include "../../include/tccg_gfx950.h"
 
 namespace tccg_lib {
@@ -37,6 +38,29 @@ constexpr TileCfg kBf1PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 32, 2, 2
                                        {64, 64, 32, 2, 2},   {192, 192, 32, 2, 2}};
 constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2},
                                       {192, 192, 128, 2, 2}};
+
+// Tiles of the vector micro-kernel (walk 3, FP32 and FP64): BM m positions
+// (a multiple of 1024: 256 threads x 4 consecutive m x MV groups) by BN n
+// positions, BK = the whole (padded) k range; WM/WN are unused.
+constexpr TileCfg kF32VecTiles[] = {{1024, 16, 8, 4, 1}, {2048, 16, 8, 4, 1}, {1024, 32, 8, 4, 1},
+                                    {1024, 8, 8, 4, 1},  {1024, 16, 16, 4, 1}, {1024, 16, 4, 4, 1}};
+constexpr TileCfg kF64VecTiles[] = {{1024, 16, 8, 4, 1}, {1024, 16, 16, 4, 1}, {1024, 16, 4, 4, 1}};
+
+constexpr int vector_count(int prec) {
+  return prec == tccg::F32 ? (int)(sizeof(kF32VecTiles) / sizeof(TileCfg))
+       : prec == tccg::F64 ? (int)(sizeof(kF64VecTiles) / sizeof(TileCfg)) : 0;
+}
+
+constexpr TileCfg vector_tile(int prec, int i) { return prec == tccg::F32 ? kF32VecTiles[i] : kF64VecTiles[i]; }
+
+inline const TileCfg* vector_tiles_of(int prec, int* n) {
+  *n = vector_count(prec);
+  switch (prec) {
+    case tccg::F32: return kF32VecTiles;
+    case tccg::F64: return kF64VecTiles;
+    default: return nullptr;
+  }
+}
 
 constexpr int packed_count(int prec) {
   return prec == tccg::F32 ? (int)(sizeof(kF32PackedTiles) / sizeof(TileCfg))
@@ -169,6 +193,26 @@ int dispatch_tile(int tile, int ma, int mb, const tccg::GettParams<typename tccg
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
+// Vector micro-kernel tile I (or a later one) of precision P.
+template <int P, int I>
+int dispatch_vector_tile(int tile, int vec, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm,
+                         hipStream_t s) {
+  if constexpr (I >= vector_count(P)) {
+    return fail(1, "invalid vector tile id");
+  } else {
+    if (tile != I) return dispatch_vector_tile<P, I + 1>(tile, vec, prm, s);
+    constexpr TileCfg tc = vector_tile(P, I);
+    const unsigned grid = (unsigned)(prm.tilesM * prm.tilesN);
+    if (vec == 4)
+      hipLaunchKernelGGL((tccg::gett_vector_kernel<P, tc.BM, tc.BN, tc.BK, 4>), dim3(grid), dim3(256), 0, s, prm);
+    else
+      hipLaunchKernelGGL((tccg::gett_vector_kernel<P, tc.BM, tc.BN, tc.BK, 1>), dim3(grid), dim3(256), 0, s, prm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
+    return 0;
+  }
+}
+
 template <int P>
 int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
              typename tccg::PrecTraits<P>::TG* C, typename tccg::PrecTraits<P>::TG alpha,
@@ -176,8 +220,12 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   if (!plan) return fail(1, "plan is NULL");
   if (!A || !B || !C) return fail(1, "tensor pointer is NULL");
   if (plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0) return fail(1, "table lengths must be positive");
+  if (plan->walk != 0 && plan->walk != 2 && plan->walk != 3) return fail(1, "walk must be 0, 2 or 3");
+  const bool vector = plan->walk == 3;
+  if (vector && P != tccg::F32 && P != tccg::F64)
+    return fail(1, "the vector micro-kernel (walk 3) computes in FP32 or FP64 only");
   int ntiles = 0;
-  const TileCfg* tiles = tiles_of(P, &ntiles);
+  const TileCfg* tiles = vector ? vector_tiles_of(P, &ntiles) : tiles_of(P, &ntiles);
   if (plan->tile < 0 || plan->tile >= ntiles) return fail(1, "invalid tile id");
   if (plan->modeA < 0 || plan->modeA > 3 || plan->modeB < 0 || plan->modeB > 3) return fail(1, "invalid load mode");
   if (!plan->offAm || !plan->offAk || !plan->offBn || !plan->offBk || !plan->offCm || !plan->offCn ||
@@ -208,7 +256,18 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
   if (plan->grid < 0) return fail(1, "grid must be >= 0");
-  if (plan->walk != 0 && plan->walk != 2) return fail(1, "walk must be 0 or 2");
+  if (vector) {
+    // modeA FREE4: groups of four m positions contiguous in A and in C
+    if (plan->Kp != tc.BK) return fail(1, "the vector micro-kernel needs the whole k range in one tile (Kp == BK)");
+    if (plan->grid != 0) return fail(1, "the vector micro-kernel runs one workgroup per tile (grid 0)");
+    const int vec = plan->modeA == TCCG_FREE4 ? 4 : 1;
+    if (vec == 4 && (!aligned16(A) || !aligned16(C)))
+      return fail(1, "the vector micro-kernel with four-wide groups needs 16-byte aligned A and C");
+    if constexpr (P == tccg::F32 || P == tccg::F64) {
+      return dispatch_vector_tile<P, 0>(plan->tile, vec, prm, (hipStream_t)stream);
+    }
+    return fail(1, "the vector micro-kernel computes in FP32 or FP64 only");
+  }
   if (plan->walk == 2) {
     if (plan->grid <= 0) return fail(1, "the streaming walk needs grid > 0");
     if (prm.kTiles != 1) return fail(1, "the streaming walk needs the whole k range in one k-step (Kp == BK)");

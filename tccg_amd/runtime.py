@@ -16,7 +16,7 @@ from typing import Optional
 import numpy as np
 
 from . import build as _build
-from .plan import BF16, BF16X3, F32, F64, FP8, PACKED_TILES, GettPlan, TILES
+from .plan import BF16, BF16X3, F32, F64, FP8, PACKED_TILES, VECTOR_TILES, GettPlan, TILES
 
 
 class GettRuntimeError(RuntimeError):
@@ -42,7 +42,8 @@ EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_
                     "tccg_batch_num_variants", "tccg_batch_variant", "tccg_gett_batch_f32",
                     "tccg_packed_workspace_bytes", "tccg_packed_tile_shape", "tccg_num_packed_tiles",
                     "tccg_gett_packed_f32", "tccg_gett_packed_f64", "tccg_gett_packed_bf16x3",
-                    "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked"]
+                    "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked",
+                    "tccg_num_vector_tiles", "tccg_vector_tile_shape"]
 
 _PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
               BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
@@ -86,6 +87,8 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
         lib.tccg_packed_workspace_bytes.restype = ctypes.c_size_t
         lib.tccg_packed_tile_shape.argtypes = lib.tccg_tile_shape.argtypes
         lib.tccg_num_packed_tiles.argtypes = [ctypes.c_int]
+        lib.tccg_vector_tile_shape.argtypes = lib.tccg_tile_shape.argtypes
+        lib.tccg_num_vector_tiles.argtypes = [ctypes.c_int]
         lib.tccg_amax_f32.argtypes = [vp, ctypes.c_int64, vp, vp]
         lib.tccg_gett_batch_f32.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
         lib.tccg_gett_batch_f32.restype = ctypes.c_int
@@ -94,7 +97,9 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
         # The Python tile tables must agree with the library's.
         for table, count, shape_of, kind in ((TILES, lib.tccg_num_tiles, lib.tccg_tile_shape, "tile"),
                                               (PACKED_TILES, lib.tccg_num_packed_tiles, lib.tccg_packed_tile_shape,
-                                               "packed tile")):
+                                               "packed tile"),
+                                              (VECTOR_TILES, lib.tccg_num_vector_tiles, lib.tccg_vector_tile_shape,
+                                               "vector tile")):
             for prec, tiles in table.items():
                 n = count(prec)
                 if n != len(tiles):
