@@ -213,6 +213,46 @@ int tccg_batch_variant(int id, int* BM, int* BN, int* BK, int* modeA, int* modeB
 int tccg_gett_batch_f32(const tccg_batch_problem* problems, const tccg_batch_work* work, int nwork, int grid,
                         void* stream);
 
+/*
+ * Tensor transposition out = alpha * permute(in) + beta * out, the step the
+ * reference's TTGT method runs before and after its GEMM (Transpose,
+ * tccg/transpose.py:74-144, executed there through HPTT's
+ * create_plan(perm, dim, alpha, in, size, outerSizeIn, beta, out,
+ * outerSizeOut, ...) / execute(); TTGT steps tccg/ttgemmt.py:133-336).
+ * Instead of a permutation the host passes separable offset tables over
+ * three index sets: X, a prefix of the input's indices (consecutive x are
+ * consecutive input elements), Y, a prefix of the output's indices, and Z,
+ * the rest; off_in(x, y, z) = inX[x] + inY[y] + inZ[z], likewise for the
+ * output.  Device uint32 tables; TCCG_INVALID marks padding of X and Y to
+ * whole tiles (not read, not written).
+ *
+ * tile > 0 (tccg_transpose_tile_shape()): TX x TY blocks through LDS,
+ * Xp % TX == 0, Yp % TY == 0.  tile == 0: input and output share their
+ * stride-one index; X is that index (Xp = its extent, contiguous in both),
+ * Yp == 1 and only inZ/outZ are read.  vec is the access width in elements:
+ * 1, or 16 / sizeof(element) when the extents of X and Y (row copy: Xp and
+ * every inZ/outZ entry) are multiples of it; then both tensors must be
+ * 16-byte aligned.  With beta == 0 `out` is not read.
+ */
+typedef struct tccg_transpose_plan {
+  const uint32_t* inX;
+  const uint32_t* outX;
+  const uint32_t* inY;
+  const uint32_t* outY;
+  const uint32_t* inZ;
+  const uint32_t* outZ;
+  int32_t Xp, Yp, Z;   /* table lengths */
+  int32_t tile;        /* 0: row copy; > 0: see tccg_transpose_tile_shape() */
+  int32_t vec;         /* elements per access: 1 or 16 / sizeof(element) */
+} tccg_transpose_plan;
+
+int tccg_num_transpose_tiles(void);
+int tccg_transpose_tile_shape(int tile, int* TX, int* TY);
+int tccg_transpose_f32(const float* in, float* out, float alpha, float beta, const tccg_transpose_plan* plan,
+                       void* stream);
+int tccg_transpose_f64(const double* in, double* out, double alpha, double beta, const tccg_transpose_plan* plan,
+                       void* stream);
+
 /* out[0] = max_i |x[i]| (device float); used for the FP8 per-tensor scales. */
 int tccg_amax_f32(const float* x, int64_t n, float* out, void* stream);
 

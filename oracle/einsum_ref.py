@@ -38,6 +38,24 @@ def contract_ref(c, a, b, A, B, C=None, alpha: float = 1.0, beta: float = 0.0) -
     return out
 
 
+def transpose_ref(in_labels: Sequence[str], out_labels: Sequence[str], X, alpha: float = 1.0, beta: float = 0.0,
+                  OUT=None) -> np.ndarray:
+    """out[out_labels] = alpha * X[in_labels] + beta * OUT in float64, TCCG
+    layout on both sides: the semantics of the reference's Transpose block
+    (tccg/transpose.py:74-144: an HPTT plan with perm[j] = position in the
+    input of output index j, getPerm, tccg/tccg_util.py:186-191, applied as
+    B = alpha * permute(A) + beta * B)."""
+    in_labels, out_labels = list(in_labels), list(out_labels)
+    rev_in = list(reversed(in_labels))
+    axes = [rev_in.index(l) for l in reversed(out_labels)]
+    out = alpha * np.transpose(np.asarray(X, dtype=np.float64), axes)
+    if beta != 0.0:
+        if OUT is None:
+            raise ValueError("beta != 0 needs OUT")
+        out = out + beta * np.asarray(OUT, dtype=np.float64)
+    return np.ascontiguousarray(out)
+
+
 def rel_frobenius(x, ref) -> float:
     """||x - ref||_F / ||ref||_F, in float64."""
     x = np.asarray(x, dtype=np.float64)

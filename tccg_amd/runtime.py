@@ -43,7 +43,9 @@ EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_
                     "tccg_packed_workspace_bytes", "tccg_packed_tile_shape", "tccg_num_packed_tiles",
                     "tccg_gett_packed_f32", "tccg_gett_packed_f64", "tccg_gett_packed_bf16x3",
                     "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked",
-                    "tccg_num_vector_tiles", "tccg_vector_tile_shape"]
+                    "tccg_num_vector_tiles", "tccg_vector_tile_shape",
+                    "tccg_num_transpose_tiles", "tccg_transpose_tile_shape", "tccg_transpose_f32",
+                    "tccg_transpose_f64"]
 
 _PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
               BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
@@ -90,6 +92,19 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
         lib.tccg_vector_tile_shape.argtypes = lib.tccg_tile_shape.argtypes
         lib.tccg_num_vector_tiles.argtypes = [ctypes.c_int]
         lib.tccg_amax_f32.argtypes = [vp, ctypes.c_int64, vp, vp]
+        from .transpose import TRANSPOSE_TILES, _CTransposePlan
+        for name, T in (("tccg_transpose_f32", f32), ("tccg_transpose_f64", f64)):
+            fn = getattr(lib, name)
+            fn.argtypes = [vp, vp, T, T, ctypes.POINTER(_CTransposePlan), vp]
+            fn.restype = ctypes.c_int
+        lib.tccg_transpose_tile_shape.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        if lib.tccg_num_transpose_tiles() != len(TRANSPOSE_TILES):
+            raise GettRuntimeError("transpose tile table mismatch")
+        for t, shape in enumerate(TRANSPOSE_TILES):
+            tx, ty = ctypes.c_int(), ctypes.c_int()
+            lib.tccg_transpose_tile_shape(t, ctypes.byref(tx), ctypes.byref(ty))
+            if (tx.value, ty.value) != shape:
+                raise GettRuntimeError("transpose tile %d shape mismatch" % t)
         lib.tccg_gett_batch_f32.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
         lib.tccg_gett_batch_f32.restype = ctypes.c_int
         lib.tccg_batch_variant.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 5
