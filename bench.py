@@ -35,7 +35,7 @@ def parse_args():
     ap.add_argument("--max-implementations", type=int, default=12,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
     ap.add_argument("--only", default=None,
-                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad (profiling)")
+                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -239,6 +239,58 @@ def run_batch(args, log):
     return res
 
 
+def run_ttgt(args, log):
+    """TTGT (the reference's transpose-transpose-GEMM-transpose method) on the
+    compute-bound contraction in FP32: the model's best candidate as a whole
+    call, and each of its transpositions alone against the HBM peak."""
+    import torch
+    from tccg_amd import Contraction
+    from tccg_amd.autotune import time_plan
+    from tccg_amd.plan import F32
+    from tccg_amd.ttgt import DeviceTtgt, best_candidates, ttgt_candidates
+    from tccg_amd.workloads import get
+    c = Contraction.from_tccg(get("abcd-aebf-dfce"))
+    cand = best_candidates(ttgt_candidates(c.problem, 4), 1)[0]
+    d = DeviceTtgt(c, cand, F32)
+    sa, sb, sc = c.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = torch.rand(sa, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    B = torch.rand(sb, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    C = torch.zeros(sc, dtype=torch.float32, device="cuda")
+    t = time_plan(d, A, B, C, 1.0, 0.0, reps=max(3, args.steps // 4))
+    C.zero_()
+    d.run(A, B, C)
+    torch.cuda.synchronize()
+    err = rel_err(C, fp64_reference(c, A, B))
+    moves = []
+    s = torch.cuda.current_stream()
+    for name, tr, src in (("A", d.tA, A), ("B", d.tB, B)):
+        if tr is None:
+            continue
+        n = tr.plan.elements()
+        dst = torch.empty(n, dtype=torch.float32, device="cuda")
+        tr.run(src.reshape(-1), dst)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(args.steps):
+            tr.run(src.reshape(-1), dst)
+        e1.record(s)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / args.steps * 1e3
+        gbs = tr.plan.moved_bytes(4) / us / 1e3
+        moves.append({"operand": name, "plan": tr.plan.describe(), "us": round(us, 2), "gbs": round(gbs, 1),
+                      "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)})
+        log("ttgt    transpose %s %8.1f us %7.0f GB/s (%.0f%% of 8 TB/s) %s"
+            % (name, us, gbs, 100 * gbs / PEAK_HBM_GBS, tr.plan.describe()))
+    res = {"workload": str(c), "candidate": "ttgemmt_" + cand.name, "gemm_plan": d.gemm_plan.describe(),
+           "call_ms": t * 1e3, "gflops": c.flops() / t / 1e9, "rel_err_vs_fp64": err, "transposes": moves,
+           "workspace_bytes": d.workspace_bytes()}
+    log("ttgt    %-18s %9.1f GFLOP/s (call %.3f ms) %s, GEMM %s" % ("abcd-aebf-dfce", res["gflops"], t * 1e3,
+                                                                res["candidate"], res["gemm_plan"]))
+    return res
+
+
 def run_grad(args, log):
     """Forward + backward (two adjoint GETTs) of C[a,b,c,d] = A[a,k,l,b] * B[l,c,k,d],
     72/32, loss 0.5*||C - T||^2, FP32; gradients against FP64 torch.einsum autograd."""
@@ -336,6 +388,7 @@ def main():
         "fp8": ("abcdef-dgfa-gceb", "fp8"),
         "batch": None,
         "grad": None,
+        "ttgt": None,
     }
     if args.only:
         forms = {args.only: forms[args.only]}
@@ -345,6 +398,8 @@ def main():
             results[key] = run_batch(args, log)
         elif key == "grad":
             results[key] = run_grad(args, log)
+        elif key == "ttgt":
+            results[key] = run_ttgt(args, log)
         else:
             results[key] = run_form(key, spec[0], spec[1], args, log)
     head = results.get("f32") or next(iter(results.values()))
@@ -359,6 +414,10 @@ def main():
         if k in results:
             details["%s_rel_err_vs_fp64" % k] = results[k].get("rel_err_vs_fp64")
     cpu = None if (args.no_cpu_baseline or args.only) else cpu_baseline(args, log)
+    if "kernel_gflops" not in head:
+        # a composite form alone (--only batch / grad / ttgt): its own record only
+        print(json.dumps({"metric": METRIC, "form": args.only, "forms": results}), flush=True)
+        return
     traffic = profiled_traffic("%s %s" % (head["workload"], head["prec"]), head["plan"])
     if head is results.get("bw"):
         # the bandwidth-bound contraction alone (--only bw): bytes against HBM

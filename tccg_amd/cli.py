@@ -11,9 +11,14 @@ model, time the best --maxImplementations of them on the GPU (printing
 "<name>: X GFLOPS // estimated: Y GFLOPS" per candidate, tccg/tccg.py:1052-1056),
 store them, and emit the fastest as tccg_implementations/gett.hip + gett.hpp
 + libgett.so exporting gett(A, B, C, alpha, beta) (tccg/tccg.py:158-186).
-The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL: ..." is printed as
-the reference does (tccg/tccg.py:426); methods other than GETT are not part
-of this backend and report 0.00.
+Unless --noTTGT is given, the TTGT candidates of minimal transposition cost
+(tccg/ttgemmt.py:133-336; tccg_amd.ttgt) are timed too and reported as the
+reference's harness reports them ("ttgemmt_<name>: X GFLOPS",
+tccg/tccg.py:1174; "Best TTGT-based implementation ...", tccg/tccg.py:399).
+The emitted entry point is always the fastest GETT: the drop-in boundary is
+the generated gett().  The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL:
+..." is printed as the reference does (tccg/tccg.py:426); the loop, GEMM and
+TCL methods are not part of this backend and report 0.00.
 
 Only --arch=gfx950 is provided; the reference's CPU/CUDA architectures are
 rejected.  --floatType=s runs FP32 (or, with --precision, the split-BF16 or
@@ -51,7 +56,7 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--noGETT", action="store_true", help="Do not use GETT implementation")
     p.add_argument("--noLoG", action="store_true", help="accepted for compatibility")
     p.add_argument("--noGEMM", action="store_true", help="accepted for compatibility")
-    p.add_argument("--noTTGT", action="store_true", help="accepted for compatibility")
+    p.add_argument("--noTTGT", action="store_true", help="do not time the TTGT method")
     p.add_argument("--useTCL", action="store_true", help="not supported by this backend")
     p.add_argument("--useDynamicMemory", action="store_true", help="accepted for compatibility")
     p.add_argument("--timePacking", action="store_true", help="accepted for compatibility")
@@ -124,7 +129,8 @@ def main(argv=None) -> int:
               % args.arch + ENDC, file=sys.stderr)
         return 1
     if args.noGETT:
-        print(FAIL + "ERROR: --noGETT leaves no method: this backend provides GETT only" + ENDC, file=sys.stderr)
+        print(FAIL + "ERROR: --noGETT leaves nothing to emit: the emitted entry point of this backend is a GETT"
+              + ENDC, file=sys.stderr)
         return 1
     try:
         prec = _precision(args)
@@ -158,6 +164,7 @@ def main(argv=None) -> int:
     chosen = plans[0]
     best_gflops = -1.0
     ref_gflops = -1.0
+    ttgt_gflops = 0.0
     gpu = False
     if not args.generateOnly:
         try:
@@ -181,6 +188,13 @@ def main(argv=None) -> int:
             best_gflops, chosen, ref_gflops = _time_candidates(args, contraction, prec, plans, db, device)
             if chosen is None:
                 return 1
+            if not args.noTTGT:
+                ttgt_gflops = _time_ttgt(args, contraction, prec)
+                if ttgt_gflops is None:
+                    return 1
+                if ttgt_gflops > best_gflops:
+                    print(WARNING + "TTGT is faster for this contraction; the emitted entry point stays the "
+                          "fastest GETT" + ENDC)
         db.close()
     print(OKGREEN + "[generate] emitting %s" % candidate_name(chosen) + ENDC)
     try:
@@ -192,8 +206,53 @@ def main(argv=None) -> int:
     print("The generated code is available at:", res["source"])
     print("Best gett-based implementation (%s) attained: %.2f GFLOPS/s" % (candidate_name(chosen), max(best_gflops, 0)))
     print("Best Loop/TTGT/GETT/reference/GEMM/TCL: %.2f / %.2f / %.2f / %.2f / %.2f / %.2f GFLOPS"
-          % (0.0, 0.0, max(best_gflops, 0.0), ref_gflops, 0.0, -1.0))
+          % (0.0, max(ttgt_gflops, 0.0), max(best_gflops, 0.0), ref_gflops, 0.0, -1.0))
     return 0
+
+
+def _time_ttgt(args, contraction, prec):
+    """Time the TTGT candidates of minimal model cost (at most
+    --maxImplementations); best GFLOP/s, or None if --testing found a wrong
+    result."""
+    import torch
+    from .autotune import time_plan
+    from .ttgt import DeviceTtgt, best_candidates, ttgt_candidates
+    esz = 8 if prec == F64 else 4
+    cands = ttgt_candidates(contraction.problem, esz)
+    best = best_candidates(cands, args.maxImplementations)
+    print("Total amount of TTGT implementations: %d" % len(cands))
+    print("Best TTGT candidates: %s" % ", ".join(c.name for c in best))
+    dt = torch.float64 if prec == F64 else torch.float32
+    sa, sb, sc = contraction.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C = C0.clone()
+    ref = _fp64_reference(contraction, A, B, C0) if args.testing else None
+    tol = {F32: 1e-5, F64: 1e-13, BF16X3: 1e-5, FP8: 8e-2}.get(prec, 1e-2)
+    top, top_name = -1.0, ""
+    for cand in best:
+        dt_ = DeviceTtgt(contraction, cand, prec)
+        name = "ttgemmt_" + cand.name
+        if ref is not None:
+            C.copy_(C0)
+            dt_.run(A, B, C)
+            torch.cuda.synchronize()
+            err = float((C.double() - ref).norm() / ref.norm())
+            if not err <= tol:
+                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+                print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
+                return None
+        C.copy_(C0)
+        t = time_plan(dt_, A, B, C, contraction.alpha, contraction.beta, reps=2 if args.fastMeasurements else 5)
+        gf = contraction.flops() / t / 1e9
+        print("%s: %.2f GFLOPS" % (name, gf))
+        if gf > top:
+            top, top_name = gf, name
+        del dt_
+    print("Best TTGT-based implementation (%s) attained: %.2f GFLOPS/s" % (top_name, max(top, 0.0)))
+    return top
 
 
 def _time_candidates(args, contraction, prec, plans, db, device):

@@ -54,7 +54,7 @@ def test_cli_double_header(tmp_path):
 @pytest.mark.parametrize("argv,err", [
     (["--arch=avx2"], "architecture"),
     (["--floatType=z"], "floatType"),
-    (["--noGETT"], "GETT only"),
+    (["--noGETT"], "nothing to emit"),
 ])
 def test_cli_rejects(tmp_path, capsys, argv, err):
     f = tmp_path / "e.tccg"

@@ -51,6 +51,7 @@ def test_cli_testing_on_gpu(tmp_path):
                           stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600)
     assert proc.returncode == 0, proc.stdout
     assert "GFLOPS // estimated:" in proc.stdout and "Best Loop/TTGT/GETT/reference/GEMM/TCL:" in proc.stdout
+    assert "Best TTGT-based implementation (ttgemmt_" in proc.stdout   # TTGT timed and checked too
     assert "ERROR" not in proc.stdout
     # second run reuses the database entry
     proc2 = subprocess.run([sys.executable, "-m", "tccg_amd", "--output", str(tmp_path / "impl2"), str(f)],
