@@ -35,7 +35,7 @@ def parse_args():
     ap.add_argument("--max-implementations", type=int, default=12,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
     ap.add_argument("--only", default=None,
-                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt (profiling)")
+                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt, log (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -291,6 +291,36 @@ def run_ttgt(args, log):
     return res
 
 
+def run_log(args, log):
+    """LoG (the reference's loop-over-GEMM method: single-index GEMMs batched
+    over the free loops, contracted loops as accumulating launches) on the
+    compute-bound contraction in FP32, the model's best candidate."""
+    import torch
+    from tccg_amd import Contraction
+    from tccg_amd.autotune import time_plan
+    from tccg_amd.loop_gemm import DeviceLoG, best_log_candidates, log_candidates
+    from tccg_amd.workloads import get
+    c = Contraction.from_tccg(get("abcd-aebf-dfce"))
+    cand = best_log_candidates(log_candidates(c.problem), 1)[0]
+    sa, sb, sc = c.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = torch.rand(sa, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    B = torch.rand(sb, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    C = torch.zeros(sc, dtype=torch.float32, device="cuda")
+    d = DeviceLoG(c, cand, A, B, C, 1.0, 0.0)
+    t = time_plan(d, A, B, C, reps=3)
+    C.zero_()
+    d.launch()
+    torch.cuda.synchronize()
+    err = rel_err(C, fp64_reference(c, A, B))
+    res = {"workload": str(c), "candidate": "loopGemm_" + cand.name, "launches": len(d.launches),
+           "problems_per_launch": len(d.layout.free_bases), "call_ms": t * 1e3, "gflops": c.flops() / t / 1e9,
+           "rel_err_vs_fp64": err}
+    log("log     %-18s %9.1f GFLOP/s (call %.3f ms, %d launches of %d GEMMs) %s"
+        % ("abcd-aebf-dfce", res["gflops"], t * 1e3, res["launches"], res["problems_per_launch"], res["candidate"]))
+    return res
+
+
 def run_grad(args, log):
     """Forward + backward (two adjoint GETTs) of C[a,b,c,d] = A[a,k,l,b] * B[l,c,k,d],
     72/32, loss 0.5*||C - T||^2, FP32; gradients against FP64 torch.einsum autograd."""
@@ -389,6 +419,7 @@ def main():
         "batch": None,
         "grad": None,
         "ttgt": None,
+        "log": None,
     }
     if args.only:
         forms = {args.only: forms[args.only]}
@@ -400,6 +431,8 @@ def main():
             results[key] = run_grad(args, log)
         elif key == "ttgt":
             results[key] = run_ttgt(args, log)
+        elif key == "log":
+            results[key] = run_log(args, log)
         else:
             results[key] = run_form(key, spec[0], spec[1], args, log)
     head = results.get("f32") or next(iter(results.values()))

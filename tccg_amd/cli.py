@@ -11,14 +11,17 @@ model, time the best --maxImplementations of them on the GPU (printing
 "<name>: X GFLOPS // estimated: Y GFLOPS" per candidate, tccg/tccg.py:1052-1056),
 store them, and emit the fastest as tccg_implementations/gett.hip + gett.hpp
 + libgett.so exporting gett(A, B, C, alpha, beta) (tccg/tccg.py:158-186).
-Unless --noTTGT is given, the TTGT candidates of minimal transposition cost
-(tccg/ttgemmt.py:133-336; tccg_amd.ttgt) are timed too and reported as the
-reference's harness reports them ("ttgemmt_<name>: X GFLOPS",
-tccg/tccg.py:1174; "Best TTGT-based implementation ...", tccg/tccg.py:399).
+Unless --noLoG / --noTTGT are given, the best LoG candidates
+(tccg/gemm_loop.py; tccg_amd.loop_gemm) and the TTGT candidates of minimal
+transposition cost (tccg/ttgemmt.py:133-336; tccg_amd.ttgt) are timed too,
+at most --maxImplementationsLoG / --maxImplementationsTTGT of each (default 1,
+as in the reference), and reported as the reference's harness reports them
+("loopGemm_<name>: X GFLOPS", "ttgemmt_<name>: X GFLOPS", tccg/tccg.py:1060-1174;
+"Best loop-based / TTGT-based implementation ...", tccg/tccg.py:377, 399).
 The emitted entry point is always the fastest GETT: the drop-in boundary is
 the generated gett().  The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL:
-..." is printed as the reference does (tccg/tccg.py:426); the loop, GEMM and
-TCL methods are not part of this backend and report 0.00.
+..." is printed as the reference does (tccg/tccg.py:426); the GEMM and TCL
+methods are not part of this backend and report 0.00 / -1.
 
 Only --arch=gfx950 is provided; the reference's CPU/CUDA architectures are
 rejected.  --floatType=s runs FP32 (or, with --precision, the split-BF16 or
@@ -50,11 +53,12 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--verbose", action="store_true", help="Verbose output (useful for debugging).")
     p.add_argument("--testing", action="store_true", help="enable testing of the generated routines.")
     p.add_argument("--keep", action="store_true", help="keep intermediate files")
-    p.add_argument("--noBatchedGEMM", action="store_true", help="accepted for compatibility (LoG is not provided)")
+    p.add_argument("--noBatchedGEMM", action="store_true",
+                   help="accepted for compatibility (LoG always runs its GEMMs as one batched launch)")
     p.add_argument("--generateOnly", action="store_true", help="only generate code (no timing on the GPU)")
     p.add_argument("--gettOnly", action="store_true", help="Only use GETT implementation (always the case here)")
     p.add_argument("--noGETT", action="store_true", help="Do not use GETT implementation")
-    p.add_argument("--noLoG", action="store_true", help="accepted for compatibility")
+    p.add_argument("--noLoG", action="store_true", help="do not time the LoG method")
     p.add_argument("--noGEMM", action="store_true", help="accepted for compatibility")
     p.add_argument("--noTTGT", action="store_true", help="do not time the TTGT method")
     p.add_argument("--useTCL", action="store_true", help="not supported by this backend")
@@ -66,8 +70,10 @@ def make_parser() -> argparse.ArgumentParser:
     p.add_argument("--numThreads", type=int, help="accepted for compatibility (one GPU per contraction)")
     p.add_argument("--maxImplementations", type=int,
                    help="maximum number of GETT candidates to evaluate (default: 16). -1 denotes infinity.")
-    p.add_argument("--maxImplementationsTTGT", type=int, help="accepted for compatibility")
-    p.add_argument("--maxImplementationsLoG", type=int, help="accepted for compatibility")
+    p.add_argument("--maxImplementationsTTGT", type=int,
+                   help="maximum number of TTGT candidates to evaluate (default: 1). -1 denotes infinity.")
+    p.add_argument("--maxImplementationsLoG", type=int,
+                   help="maximum number of LoG candidates to evaluate (default: 1). -1 denotes infinity.")
     p.add_argument("--maxWorkspace", type=float, help="accepted for compatibility")
     p.add_argument("--arch", metavar="arch", type=str, help="architecture: gfx950 (default)")
     p.add_argument("--compiler", metavar="compiler", type=str, help="accepted for compatibility (hipcc is used)")
@@ -120,10 +126,10 @@ def _precision(args) -> int:
 def main(argv=None) -> int:
     args = make_parser().parse_args(argv)
     args.arch = args.arch or "gfx950"
-    if args.maxImplementations is None:
-        args.maxImplementations = 16
-    elif args.maxImplementations == -1:
-        args.maxImplementations = 1000000000
+    # defaults of TccgArgs (tccg/tccg.py:40-42); -1 denotes infinity
+    for name, default in (("maxImplementations", 16), ("maxImplementationsTTGT", 1), ("maxImplementationsLoG", 1)):
+        v = getattr(args, name)
+        setattr(args, name, default if v is None else (1000000000 if v == -1 else v))
     if args.arch != "gfx950":
         print(FAIL + "[GETT] Error: architecture %s is not provided by this backend (use --arch=gfx950)"
               % args.arch + ENDC, file=sys.stderr)
@@ -165,6 +171,7 @@ def main(argv=None) -> int:
     best_gflops = -1.0
     ref_gflops = -1.0
     ttgt_gflops = 0.0
+    log_gflops = 0.0
     gpu = False
     if not args.generateOnly:
         try:
@@ -188,13 +195,18 @@ def main(argv=None) -> int:
             best_gflops, chosen, ref_gflops = _time_candidates(args, contraction, prec, plans, db, device)
             if chosen is None:
                 return 1
+            if not args.noLoG:
+                log_gflops = _time_log(args, contraction, prec)
+                if log_gflops is None:
+                    return 1
             if not args.noTTGT:
                 ttgt_gflops = _time_ttgt(args, contraction, prec)
                 if ttgt_gflops is None:
                     return 1
-                if ttgt_gflops > best_gflops:
-                    print(WARNING + "TTGT is faster for this contraction; the emitted entry point stays the "
-                          "fastest GETT" + ENDC)
+            other = max(log_gflops, ttgt_gflops)
+            if other > best_gflops:
+                print(WARNING + "%s is faster for this contraction; the emitted entry point stays the fastest GETT"
+                      % ("LoG" if log_gflops >= ttgt_gflops else "TTGT") + ENDC)
         db.close()
     print(OKGREEN + "[generate] emitting %s" % candidate_name(chosen) + ENDC)
     try:
@@ -206,8 +218,60 @@ def main(argv=None) -> int:
     print("The generated code is available at:", res["source"])
     print("Best gett-based implementation (%s) attained: %.2f GFLOPS/s" % (candidate_name(chosen), max(best_gflops, 0)))
     print("Best Loop/TTGT/GETT/reference/GEMM/TCL: %.2f / %.2f / %.2f / %.2f / %.2f / %.2f GFLOPS"
-          % (0.0, max(ttgt_gflops, 0.0), max(best_gflops, 0.0), ref_gflops, 0.0, -1.0))
+          % (max(log_gflops, 0.0), max(ttgt_gflops, 0.0), max(best_gflops, 0.0), ref_gflops, 0.0, -1.0))
     return 0
+
+
+def _test_inputs(contraction, prec, with_ref: bool):
+    import torch
+    dt = torch.float64 if prec == F64 else torch.float32
+    sa, sb, sc = contraction.shapes()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    ref = _fp64_reference(contraction, A, B, C0) if with_ref else None
+    return A, B, C0, ref
+
+
+def _time_log(args, contraction, prec):
+    """Time the LoG candidates with the largest GEMM and batch (at most
+    --maxImplementationsLoG; FP32 only: the batched launch computes in FP32).
+    Best GFLOP/s (0 when LoG does not apply), or None if --testing found a
+    wrong result."""
+    import torch
+    from .autotune import time_plan
+    from .loop_gemm import DeviceLoG, best_log_candidates, log_candidates
+    cands = log_candidates(contraction.problem)
+    if cands is None or prec != F32:
+        print("LoG not possible" if cands is None else "LoG: FP32 only (skipped for %s)" % PREC_NAMES[prec])
+        return 0.0
+    best = best_log_candidates(cands, args.maxImplementationsLoG)
+    print("Total amount of LoG implementations: %d" % len(cands))
+    print("Best LoG candidates: %s" % ", ".join(c.name for c in best))
+    A, B, C0, ref = _test_inputs(contraction, prec, args.testing)
+    C = C0.clone()
+    top, top_name = -1.0, ""
+    for cand in best:
+        d = DeviceLoG(contraction, cand, A, B, C)
+        name = "loopGemm_" + cand.name
+        if ref is not None:
+            C.copy_(C0)
+            d.launch()
+            torch.cuda.synchronize()
+            err = float((C.double() - ref).norm() / ref.norm())
+            if not err <= 1e-5:
+                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+                print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
+                return None
+        C.copy_(C0)
+        t = time_plan(d, A, B, C, contraction.alpha, contraction.beta, reps=2 if args.fastMeasurements else 5)
+        gf = contraction.flops() / t / 1e9
+        print("%s: %.2f GFLOPS" % (name, gf))
+        if gf > top:
+            top, top_name = gf, name
+    print("Best loop-based implementation (%s) attained: %.2f GFLOPS/s" % (top_name, max(top, 0.0)))
+    return top
 
 
 def _time_ttgt(args, contraction, prec):
@@ -219,17 +283,11 @@ def _time_ttgt(args, contraction, prec):
     from .ttgt import DeviceTtgt, best_candidates, ttgt_candidates
     esz = 8 if prec == F64 else 4
     cands = ttgt_candidates(contraction.problem, esz)
-    best = best_candidates(cands, args.maxImplementations)
+    best = best_candidates(cands, args.maxImplementationsTTGT)
     print("Total amount of TTGT implementations: %d" % len(cands))
     print("Best TTGT candidates: %s" % ", ".join(c.name for c in best))
-    dt = torch.float64 if prec == F64 else torch.float32
-    sa, sb, sc = contraction.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    A, B, C0, ref = _test_inputs(contraction, prec, args.testing)
     C = C0.clone()
-    ref = _fp64_reference(contraction, A, B, C0) if args.testing else None
     tol = {F32: 1e-5, F64: 1e-13, BF16X3: 1e-5, FP8: 8e-2}.get(prec, 1e-2)
     top, top_name = -1.0, ""
     for cand in best:
