@@ -47,26 +47,35 @@ PATTERNS = [
 ]
 
 
+def gather_plan(c, prec):
+    """The model's best plan of the gather kernel (one workgroup per tile)."""
+    from tccg_amd.plan import candidate_plans
+    return next(p for p in candidate_plans(c.problem, prec) if not p.packed and p.walk == 0 and not p.grid)
+
+
 @pytest.mark.parametrize("pi", range(len(PATTERNS)))
 @pytest.mark.parametrize("ext", [(5, 7, 3), (8, 12, 16), (33, 17, 65)])
 @pytest.mark.parametrize("prec", [F32, F64, BF16X3])
 def test_patterns_ragged(pi, ext, prec):
+    """The default plan (whatever kernel the model prefers) and the gather
+    kernel's plan, whose load modes these patterns cover."""
     c_l, a_l, b_l = PATTERNS[pi]
     labels = sorted(set(c_l + a_l + b_l))
     sizes = {l: ext[i % len(ext)] for i, l in enumerate(labels)}
     c = Contraction.from_labels(c_l, a_l, b_l, sizes)
     check(c, prec, seed=pi)
+    check(c, prec, seed=pi, plan=gather_plan(c, prec))
 
 
 def test_load_modes_covered():
-    """The pattern list reaches every load mode for A and B."""
+    """The pattern list reaches every load mode of the gather kernel for A and B."""
     seen = set()
     for c_l, a_l, b_l in PATTERNS:
         for ext in [(8, 12, 16), (5, 7, 3)]:
             labels = sorted(set(c_l + a_l + b_l))
             sizes = {l: ext[i % len(ext)] for i, l in enumerate(labels)}
             c = Contraction.from_labels(c_l, a_l, b_l, sizes)
-            p = c.plan(F32)
+            p = gather_plan(c, F32)
             seen.add(("A", p.modeA))
             seen.add(("B", p.modeB))
     for op in "AB":
