@@ -32,7 +32,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--max-implementations", type=int, default=12,
+    ap.add_argument("--max-implementations", type=int, default=16,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
     ap.add_argument("--only", default=None,
                     help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt, log (profiling)")
@@ -361,19 +361,20 @@ def run_grad(args, log):
     return out
 
 
-def profiled_traffic(key: str, plan: str):
+def profiled_traffic(workload: str, prec: str, plan: str):
     """HBM bytes per launch of the dominant kernel from profiles/traffic.json
-    (calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE of the same workload and
-    plan); None when no profile of this exact plan is committed."""
+    (calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE of the same workload,
+    precision and plan); None when no profile of this exact plan is committed."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            entry = json.load(f).get(key)
+            entries = json.load(f)
     except (OSError, ValueError):
         return None
-    if not entry or entry.get("plan") != plan:
-        return None
-    return entry.get("hbm_bytes_per_launch")
+    for e in entries:
+        if e.get("workload") == workload and e.get("prec") == prec and e.get("plan") == plan:
+            return e.get("hbm_bytes_per_launch")
+    return None
 
 
 def cpu_baseline(args, log):
@@ -451,7 +452,7 @@ def main():
         # a composite form alone (--only batch / grad / ttgt): its own record only
         print(json.dumps({"metric": METRIC, "form": args.only, "forms": results}), flush=True)
         return
-    traffic = profiled_traffic("%s %s" % (head["workload"], head["prec"]), head["plan"])
+    traffic = profiled_traffic(head["workload"], head["prec"], head["plan"])
     if head is results.get("bw"):
         # the bandwidth-bound contraction alone (--only bw): bytes against HBM
         roofline = {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
