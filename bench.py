@@ -251,7 +251,7 @@ def run_ttgt(args, log):
     from tccg_amd.workloads import get
     c = Contraction.from_tccg(get("abcd-aebf-dfce"))
     cand = best_candidates(ttgt_candidates(c.problem, 4), 1)[0]
-    d = DeviceTtgt(c, cand, F32)
+    d = DeviceTtgt(c, cand, F32, tune=True)
     sa, sb, sc = c.shapes()
     g = torch.Generator(device="cuda").manual_seed(0)
     A = torch.rand(sa, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
@@ -340,7 +340,7 @@ def run_grad(args, log):
         def step():
             A1.grad = None
             B1.grad = None
-            L = 0.5 * ((contract(A1, B1, c, prec) - T) ** 2).sum()
+            L = 0.5 * ((contract(A1, B1, c, prec, tune=True) - T) ** 2).sum()
             L.backward()
         for _ in range(max(1, args.warmup)):
             step()

@@ -73,12 +73,17 @@ class GettFunction:
         return cls._fn
 
 
-def contract(A, B, contraction: Contraction, prec="f32"):
+def contract(A, B, contraction: Contraction, prec="f32", tune: bool = False):
     """alpha * contraction(A, B) as a differentiable torch op (C is a new tensor;
-    the contraction's beta is not applied)."""
+    the contraction's beta is not applied).  tune=True times the candidate
+    plans of the forward and of both adjoint contractions on first use
+    (Contraction.tune) and keeps the fastest."""
     p = _prec(prec)
     import torch
     want = torch.float64 if p == F64 else torch.float32
     if A.dtype != want or B.dtype != want:
         raise TypeError("precision %s expects %s inputs" % (prec, want))
+    if tune and p not in contraction._tuned:
+        for c in (contraction,) + adjoints(contraction):
+            c.tune(p)
     return GettFunction.get().apply(A, B, contraction, p)

@@ -291,7 +291,7 @@ def _time_ttgt(args, contraction, prec):
     tol = {F32: 1e-5, F64: 1e-13, BF16X3: 1e-5, FP8: 8e-2}.get(prec, 1e-2)
     top, top_name = -1.0, ""
     for cand in best:
-        dt_ = DeviceTtgt(contraction, cand, prec)
+        dt_ = DeviceTtgt(contraction, cand, prec, tune=True)
         name = "ttgemmt_" + cand.name
         if ref is not None:
             C.copy_(C0)

@@ -53,6 +53,7 @@ class Contraction:
         self.alpha, self.beta = float(alpha), float(beta)
         self.problem = GettProblem(self.C, self.A, self.B)
         self._device_plans: Dict[Tuple, object] = {}
+        self._tuned: Dict[int, GettPlan] = {}
 
     @classmethod
     def from_tccg(cls, text: str, fuse: bool = True) -> "Contraction":
@@ -76,13 +77,26 @@ class Contraction:
         return self.A0.torch_shape(), self.B0.torch_shape(), self.C0.torch_shape()
 
     def plan(self, prec="f32", tile: Optional[int] = None, m_blocked: Optional[bool] = None) -> GettPlan:
+        """The tuned plan (after tune()), else the model's best; an explicit
+        tile / blocking builds that plan of the gather kernel."""
         p = _prec_id(prec)
         if tile is None and m_blocked is None:
-            return default_plan(self.problem, p)
+            return self._tuned.get(p) or default_plan(self.problem, p)
         if tile is None:
             # `tile` indexes the gather kernel's table: the model's best plan of that kernel
             tile = next(q.tile for q in candidate_plans(self.problem, p) if not q.packed and q.walk != 3)
         return build_plan(self.problem, p, tile, m_blocked=m_blocked)
+
+    def tune(self, prec="f32", max_implementations: int = 16, inputs=None):
+        """Time the best-ranked candidate plans on the GPU (tccg_amd.autotune,
+        the reference's measure-and-select step, tccg/tccg.py:188-430) and make
+        the fastest this contraction's plan for `prec`; returns its measurement."""
+        from .autotune import autotune
+        p = _prec_id(prec)
+        best = autotune(self, p, max_implementations=max_implementations, inputs=inputs)[0]
+        self._tuned[p] = best.plan
+        self._device_plans.pop((p, None, None), None)
+        return best
 
     def device_plan(self, prec="f32", tile: Optional[int] = None, m_blocked: Optional[bool] = None,
                     plan: Optional[GettPlan] = None):

@@ -118,7 +118,8 @@ class DeviceTtgt:
     """One TTGT candidate ready to run on the GPU: transposition plans, the
     GEMM's GETT plan and the workspace for the temporaries."""
 
-    def __init__(self, contraction, cand: TtgtCandidate, prec: int = F32, gemm_plan=None, device=None):
+    def __init__(self, contraction, cand: TtgtCandidate, prec: int = F32, gemm_plan=None, device=None,
+                 tune: bool = False):
         from .contraction import Contraction
         from .runtime import _check_cuda
         from .transpose import DeviceTranspose, plan_transpose
@@ -146,6 +147,9 @@ class DeviceTtgt:
         self.tC = (DeviceTranspose(plan_transpose(cand.idxC, pb.C.labels(), sizes, elem_bytes=esz), dev)
                    if cand.transC else None)
         self.gemm = Contraction.from_labels(cand.idxC, cand.idxA, cand.idxB, sizes)
+        if tune and gemm_plan is None:
+            # the reference's GEMM is a vendor BLAS call; here the GEMM's GETT plans are timed
+            self.gemm.tune(prec)
         self.gemm_plan = gemm_plan if gemm_plan is not None else self.gemm.plan(prec)
         self.gemm_dp = self.gemm.device_plan(plan=self.gemm_plan)
         self._sizes = sizes

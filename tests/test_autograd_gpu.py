@@ -80,3 +80,24 @@ def test_full_size_gradients_vs_torch(prec, tol):
     for g1, g2 in ((A1.grad, A2.grad), (B1.grad, B2.grad)):
         err = ((g1.double() - g2).norm() / g2.norm()).item()
         assert err < tol, "%s gradient rel. error %.3e" % (prec, err)
+
+
+def test_tuned_contract_gradients():
+    """tune=True: forward and both adjoints get the autotuner's fastest plans;
+    gradients stay those of the FP64 reference."""
+    import torch
+    from tccg_amd.autograd import adjoints
+    c = Contraction.from_tccg(scaled("abcd-aklb-lckd", 0.25))
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    A = torch.randn(c.A0.torch_shape(), dtype=torch.float32, device="cuda", generator=gen)
+    B = torch.randn(c.B0.torch_shape(), dtype=torch.float32, device="cuda", generator=gen)
+    T = torch.randn(c.C0.torch_shape(), dtype=torch.float32, device="cuda", generator=gen)
+    A1, B1 = A.clone().requires_grad_(), B.clone().requires_grad_()
+    L = 0.5 * ((contract(A1, B1, c, "f32", tune=True) - T) ** 2).sum()
+    L.backward()
+    assert all(0 in x._tuned for x in (c,) + adjoints(c))
+    A2, B2 = A.double().requires_grad_(), B.double().requires_grad_()
+    L2 = 0.5 * ((torch.einsum(einsum_eq(c), A2, B2) - T.double()) ** 2).sum()
+    L2.backward()
+    for g1, g2 in ((A1.grad, A2.grad), (B1.grad, B2.grad)):
+        assert ((g1.double() - g2).norm() / g2.norm()).item() < 1e-5
