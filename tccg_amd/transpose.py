@@ -162,8 +162,13 @@ def plan_transpose(in_labels: Sequence[str], out_labels: Sequence[str], sizes: D
     if groups[0][0] == first_out or tile == 0:
         if groups[0][0] != first_out:
             raise ValueError("the row copy needs input and output to share their stride-one index")
-        # rows in output order: consecutive rows are written back to back
-        X, Y, Z = [groups[0]], [], sorted(groups[1:], key=lambda g: out_labels.index(g[0]))
+        # Rows of at least 1 KiB are walked in output order (consecutive rows
+        # written back to back), shorter ones in input order (read back to
+        # back): measured on MI355X, 2.3 KB rows move 7 % faster in output
+        # order, 256-288 B rows 2-14 % faster in input order.
+        row_bytes = elem_bytes * _prod(sizes[l] for l in groups[0])
+        rest = groups[1:] if row_bytes < 1024 else sorted(groups[1:], key=lambda g: out_labels.index(g[0]))
+        X, Y, Z = [groups[0]], [], rest
         inZ = _offsets(Z, sizes, sin, 1)
         outZ = _offsets(Z, sizes, sout, 1)
         nx = _prod(sizes[l] for l in X[0])
