@@ -29,8 +29,9 @@ class Measurement:
 def time_plan(dp, A, B, C, alpha=1.0, beta=0.0, reps: int = 5, warmup: int = 1) -> float:
     """Launch time (s) of a DevicePlan: the minimum over `reps` groups of the
     average of back-to-back launches, timed with HIP events on the launching
-    stream (a group spans about 0.2 ms, so host launch latency does not count
-    for short kernels)."""
+    stream (a group spans about 2 ms: host launch latency does not count for
+    short kernels, and near-equal candidates are told apart by averages over
+    several launches rather than single ones)."""
     import torch
     s = torch.cuda.current_stream()
     for _ in range(warmup):
@@ -41,7 +42,7 @@ def time_plan(dp, A, B, C, alpha=1.0, beta=0.0, reps: int = 5, warmup: int = 1) 
     dp.run(A, B, C, alpha, beta)
     e1.record(s)
     e1.synchronize()
-    n = max(1, min(20, int(2e-4 / max(e0.elapsed_time(e1) * 1e-3, 1e-7)) + 1))
+    n = max(1, min(50, int(2e-3 / max(e0.elapsed_time(e1) * 1e-3, 1e-7)) + 1))
     best = float("inf")
     for _ in range(reps):
         e0.record(s)
