@@ -55,7 +55,8 @@ def time_plan(dp, A, B, C, alpha=1.0, beta=0.0, reps: int = 5, warmup: int = 1) 
 
 
 def autotune(contraction, prec: int, max_implementations: int = 16, reps: int = 5, inputs=None,
-             candidates: Optional[Sequence[GettPlan]] = None, verbose: bool = False) -> List[Measurement]:
+             candidates: Optional[Sequence[GettPlan]] = None, verbose: bool = False,
+             rounds: int = 2) -> List[Measurement]:
     """Time up to max_implementations model-ranked candidates; fastest first."""
     import torch
     from .runtime import DevicePlan
@@ -71,10 +72,17 @@ def autotune(contraction, prec: int, max_implementations: int = 16, reps: int = 
         C = torch.zeros(sc, dtype=dt, device="cuda")
     else:
         A, B, C = inputs
+    # Two interleaved rounds, the minimum per candidate: clocks drift under
+    # the power limit, and a single pass in a fixed order would favour
+    # whichever candidates happen to run at the better clock.
+    best = [float("inf")] * len(plans)
+    for _ in range(rounds):
+        for i, p in enumerate(plans):
+            dp = DevicePlan(p)
+            best[i] = min(best[i], time_plan(dp, A, B, C, reps=reps))
+            del dp
     out = []
-    for p in plans:
-        dp = DevicePlan(p)
-        t = time_plan(dp, A, B, C, reps=reps)
+    for p, t in zip(plans, best):
         m = Measurement(p, t, estimate_seconds(p))
         if verbose:
             print("  %-100s %9.1f us  %8.1f GFLOP/s (estimated %.1f us)"
