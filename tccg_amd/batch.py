@@ -20,7 +20,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from .contraction import Contraction
-from .plan import F32, TILES, build_plan
+from .plan import F32, PACKED_TILES, TILES, build_packed_plan, build_plan
 
 # batch shape id -> tile id of TILES[F32] with the same (BM, BN, BK)
 BATCH_SHAPES = {0: TILES[F32].index((64, 64, 32)), 1: TILES[F32].index((128, 128, 32))}
@@ -39,17 +39,33 @@ def pattern_key(c: Contraction) -> str:
     return "%s=%s*%s" % (c.C0.index_str(), c.A0.index_str(), c.B0.index_str())
 
 
+# packed tile of the problems that run on the packed path (FP32 128 x 128 x 16)
+LARGE_TILE = PACKED_TILES[F32].index((128, 128, 16))
+
+
 class GettBatch:
-    """Fixed set of contractions bound to fixed GPU tensors, launched together."""
+    """Fixed set of contractions bound to fixed GPU tensors, launched together.
+
+    packed_min_tiles: problems whose packed plan (128 x 128 tiles) has at least
+    this many C tiles, enough to fill the GPU on their own, run one after the
+    other on the packed path (pack kernels + DMA-ring GETT, sharing one
+    workspace) on the same stream, ahead of the single persistent launch that
+    runs all other problems.  None: every problem in the persistent launch."""
 
     def __init__(self, contractions: Sequence[Contraction], shapes: Optional[Dict[str, int]] = None,
-                 alphas: Optional[Sequence[float]] = None, betas: Optional[Sequence[float]] = None):
+                 alphas: Optional[Sequence[float]] = None, betas: Optional[Sequence[float]] = None,
+                 packed_min_tiles: Optional[int] = None):
         self.contractions = list(contractions)
         self.shapes = dict(shapes or {})
         self.alphas = list(alphas) if alphas is not None else [c.alpha for c in self.contractions]
         self.betas = list(betas) if betas is not None else [c.beta for c in self.contractions]
         self.plans = []
-        for c in self.contractions:
+        self.large: Dict[int, object] = {}      # problem index -> packed plan
+        for i, c in enumerate(self.contractions):
+            if packed_min_tiles is not None:
+                pp = build_packed_plan(c.problem, F32, LARGE_TILE)
+                if pp.ntiles() >= packed_min_tiles:
+                    self.large[i] = pp
             sh = self.shapes.get(pattern_key(c), self._default_shape(c))
             self.plans.append(build_plan(c.problem, F32, BATCH_SHAPES[sh]))
         self._bound = None
@@ -66,11 +82,20 @@ class GettBatch:
         pointers are baked in; reuse the same tensors for every launch)."""
         import torch
         from .runtime import load_library
+        from .runtime import DevicePlan
         lib = load_library()
         dev = device or As[0].device
+        # problems on the packed path: one shared workspace (they run one after the other)
+        large = []
+        if self.large:
+            ws = max(p.workspace_bytes() for p in self.large.values())
+            shared = torch.empty(ws, dtype=torch.uint8, device=dev)
+            for i, p in self.large.items():
+                large.append((DevicePlan(p, device=dev, workspace=shared), i))
+        small = [i for i in range(len(self.plans)) if i not in self.large]
         # all tables of all problems in one int32 buffer
         chunks, offsets, pos = [], [], 0
-        for p in self.plans:
+        for p in (self.plans[i] for i in small):
             offs = []
             for arr in (p.offAm, p.offAk, p.offBn, p.offBk, p.offCm, p.offCn, p.ordA, p.ordB):
                 a = np.ascontiguousarray(arr).view(np.int32)
@@ -82,20 +107,22 @@ class GettBatch:
                     chunks.append(np.zeros(pad, np.int32))
                     pos += pad
             offsets.append(offs)
-        tables = torch.from_numpy(np.concatenate(chunks)).to(dev)
+        tables = torch.from_numpy(np.concatenate(chunks) if chunks else np.zeros(4, np.int32)).to(dev)
         base = tables.data_ptr()
-        probs = (_CProblem * len(self.plans))()
+        probs = (_CProblem * max(1, len(small)))()
         work = []
-        for i, (p, c, A, B, C) in enumerate(zip(self.plans, self.contractions, As, Bs, Cs)):
+        for j in range(len(small)):
+            i = small[j]
+            p, c, A, B, C = self.plans[i], self.contractions[i], As[i], Bs[i], Cs[i]
             for name, t, T in (("A", A, c.A0), ("B", B, c.B0), ("C", C, c.C0)):
                 if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size():
                     raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements"
                                      % (i, name, T.size()))
             opA, opB = (B, A) if p.problem.swapped else (A, B)
-            q = probs[i]
+            q = probs[j]
             q.A, q.B, q.C = opA.data_ptr(), opB.data_ptr(), C.data_ptr()
             (q.offAm, q.offAk, q.offBn, q.offBk, q.offCm, q.offCn, q.ordA, q.ordB) = \
-                [base + 4 * o for o in offsets[i]]
+                [base + 4 * o for o in offsets[j]]
             q.tilesM, q.tilesN, q.kTiles = p.Mp // p.BM, p.Np // p.BN, p.Kp // p.BK
             sh = [k for k, v in BATCH_SHAPES.items() if v == p.tile][0]
             q.variant = sh * 16 + p.modeA * 4 + p.modeB
@@ -103,17 +130,17 @@ class GettBatch:
             cost = p.BM * p.BN * q.kTiles
             for tm in range(q.tilesM):
                 for tn in range(q.tilesN):
-                    work.append((cost, i, tm, tn))
+                    work.append((cost, j, tm, tn))   # j: row of the problem table
         # longest tiles first: the round-robin walk then approximates LPT
         work.sort(key=lambda w: -w[0])
-        wl = np.array([[w[1], w[2], w[3], 0] for w in work], dtype=np.int32)
+        wl = np.array([[w[1], w[2], w[3], 0] for w in work], dtype=np.int32).reshape(-1, 4)
         pbytes = np.frombuffer(bytes(probs), dtype=np.uint8)
         dprobs = torch.from_numpy(pbytes.copy()).to(dev)
         dwork = torch.from_numpy(wl).to(dev)
         if grid is None:
-            grid = min(len(work), 256 * 2)
+            grid = max(1, min(len(work), 256 * 2))
         self._bound = dict(tables=tables, probs=dprobs, work=dwork, nwork=len(work), grid=int(grid),
-                           tensors=(list(As), list(Bs), list(Cs)), lib=lib)
+                           tensors=(list(As), list(Bs), list(Cs)), lib=lib, large=large)
         return self
 
     def launch(self, stream=None) -> None:
@@ -123,6 +150,11 @@ class GettBatch:
         if b is None:
             raise RuntimeError("bind() the batch to tensors first")
         s = stream if stream is not None else torch.cuda.current_stream()
+        As, Bs, Cs = b["tensors"]
+        for dp, i in b["large"]:   # one after the other: they share the workspace
+            dp.run(As[i], Bs[i], Cs[i], self.alphas[i], self.betas[i], stream=s)
+        if b["nwork"] == 0:
+            return
         rc = b["lib"].tccg_gett_batch_f32(ctypes.c_void_p(b["probs"].data_ptr()), ctypes.c_void_p(b["work"].data_ptr()),
                                           b["nwork"], b["grid"], ctypes.c_void_p(s.cuda_stream))
         if rc != 0:

@@ -148,7 +148,10 @@ def _check_cuda():
 class DevicePlan:
     """A GettPlan whose tables live on the GPU."""
 
-    def __init__(self, plan: GettPlan, device=None):
+    def __init__(self, plan: GettPlan, device=None, workspace=None):
+        """workspace: for packed plans, a device uint8 tensor of at least the
+        plan's workspace bytes to use instead of a new one (callers that run
+        several packed plans one after the other can share one)."""
         torch = _check_cuda()
         self.plan = plan
         dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -176,7 +179,13 @@ class DevicePlan:
             nbytes = plan.workspace_bytes()
             if load_library().tccg_packed_workspace_bytes(plan.prec, ctypes.byref(c)) != nbytes:
                 raise GettRuntimeError("packed workspace size disagrees with the library")
-            self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            if workspace is not None:
+                if workspace.dtype != torch.uint8 or workspace.numel() < nbytes or workspace.device != dev:
+                    raise GettRuntimeError("shared workspace must be a uint8 tensor of >= %d bytes on %s"
+                                           % (nbytes, dev))
+                self.workspace = workspace
+            else:
+                self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=dev)
 
     def _operands(self, A, B, C):
         """Tensors in the .tccg's (A, B, C) order -> kernel operands (opA, opB)."""

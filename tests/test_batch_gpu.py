@@ -78,3 +78,30 @@ def test_batch_graph_replays():
         torch.cuda.synchronize()
         for C, E in zip(Cs, eager):
             assert torch.equal(C, E), "replay %d differs from the eager launch" % r
+
+
+def test_batch_with_packed_problems():
+    """Problems with enough tiles run on the packed path (shared workspace)
+    ahead of the persistent launch; every problem still matches the oracle,
+    eagerly and in graph replays."""
+    import torch
+    cs = random_batch(40, seed=11, hi=64)
+    As, Bs, Cs, hA, hB = make_tensors(cs, 4)
+    b = GettBatch(cs, packed_min_tiles=16)
+    assert 0 < len(b.large) < len(cs)
+    b.bind(As, Bs, Cs)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, c in enumerate(cs):
+        ref = contract_ref(lab(c.C0), lab(c.A0), lab(c.B0), hA[i], hB[i])
+        assert rel_frobenius(Cs[i].cpu().numpy(), ref) < 1e-5, (i, i in b.large, str(c))
+    eager = [C.clone() for C in Cs]
+    g = b.capture()
+    for r in range(3):
+        for C in Cs:
+            C.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for C, E in zip(Cs, eager):
+            assert torch.equal(C, E), "replay %d differs from the eager launch" % r
