@@ -147,14 +147,15 @@ def kernel_only(dp, C, steps: int):
     return e0.elapsed_time(e1) / steps * 1e-3
 
 
-def cold_min(dp, A, B, C, reps: int = 5) -> float:
-    """Minimum launch time after writing a 512 MiB buffer (evicts L2 and the
-    256 MiB Infinity Cache), the reference's trash-then-time method
-    (tccg/tccg.py:725-737, tccg/tccg.py:1032-1058)."""
+def cold_times(dp, A, B, C, reps: int = 10):
+    """(minimum, median) call time after writing a 512 MiB buffer before each
+    call (evicts L2 and the 256 MiB Infinity Cache), the reference's
+    trash-then-time method (tccg/tccg.py:725-737, tccg/tccg.py:1032-1058)."""
+    import statistics
     import torch
     trash = torch.empty(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
     s = torch.cuda.current_stream()
-    best = float("inf")
+    times = []
     for r in range(reps):
         trash.fill_(float(r))
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -162,9 +163,9 @@ def cold_min(dp, A, B, C, reps: int = 5) -> float:
         dp.run(A, B, C)
         e1.record(s)
         e1.synchronize()
-        best = min(best, e0.elapsed_time(e1) * 1e-3)
+        times.append(e0.elapsed_time(e1) * 1e-3)
     del trash
-    return best
+    return min(times), statistics.median(times)
 
 
 def run_form(name, wl, prec, args, log):
@@ -206,14 +207,14 @@ def run_form(name, wl, prec, args, log):
             kern = kernel_only(dp, C, args.steps)
     if kern is None:
         kern = call
-    cold = cold_min(dp, A, B, C)
+    cold, cold_med = cold_times(dp, A, B, C)
     flops = c.flops()
     bytes_ = A.element_size() * (A.numel() + B.numel() + C.numel())
     res = {
         "workload": str(c), "prec": prec, "M": c.problem.M, "N": c.problem.N, "K": c.problem.K,
         "plan": best.describe(), "tuned_candidates": len(ms), "tune_seconds": round(tune_s, 2),
         "wall_ms_per_step": wall / args.steps * 1e3, "call_ms": call * 1e3, "kernel_ms": kern * 1e3,
-        "cold_min_ms": cold * 1e3,
+        "cold_min_ms": cold * 1e3, "cold_median_ms": cold_med * 1e3,
         "gflops": flops / (wall / args.steps) / 1e9, "kernel_gflops": flops / kern / 1e9,
         "kernel_gbs": bytes_ / kern / 1e9, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
     }
