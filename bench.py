@@ -35,7 +35,7 @@ def parse_args():
     ap.add_argument("--max-implementations", type=int, default=16,
                     help="candidates timed by the autotuner per workload (tccg --maxImplementations)")
     ap.add_argument("--only", default=None,
-                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt, log (profiling)")
+                    help="run one form only: f32, bw, f64, bf16x3, bf16, fp8, fp8_f32, batch, grad, ttgt, log, mfma (profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-seconds", type=float, default=15.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -341,6 +341,55 @@ def run_ttgt(args, log):
     return res
 
 
+def run_mfma(args, log):
+    """Sustained MFMA rates of this chip (tccg_amd/csrc/mfma_probe.hip: every
+    SIMD busy with independent register-only MFMAs): FP32 32x32x2, BF16
+    32x32x16 and FP64 16x16x4.  A 32x32x16 block of a contraction takes 8 FP32
+    MFMAs or 6 BF16 ones (the three-way split), so (BF16 rate / 6) / FP32 rate
+    bounds the split-BF16 / FP32 throughput ratio of any GETT kernel."""
+    import ctypes
+    import torch
+    from tccg_amd import build as hipbuild
+    path = hipbuild.probe_library_path()
+    if not path.exists():
+        log("mfma probe library missing (build() compiles it)")
+        return None
+    lib = ctypes.CDLL(str(path))
+    lib.tccg_probe_mfma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.tccg_probe_mfma_flops.restype = ctypes.c_double
+    lib.tccg_probe_mfma_round.argtypes = [ctypes.c_int]
+    s = torch.cuda.current_stream()
+    res = {}
+    for kind, name, iters in ((0, "f32_32x32x2", 1000), (1, "bf16_32x32x16", 2000), (2, "f64_16x16x4", 1000)):
+        rnd = lib.tccg_probe_mfma_round(kind)
+        if rnd <= 0:
+            raise RuntimeError("mfma probe: no resident workgroups")
+        blocks = 4 * rnd   # four whole rounds: every SIMD equally busy to the end
+        out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+
+        def launch():
+            if lib.tccg_probe_mfma(kind, blocks, iters, ctypes.c_void_p(out.data_ptr()),
+                                   ctypes.c_void_p(s.cuda_stream)) != 0:
+                raise RuntimeError("mfma probe launch failed")
+        launch()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            launch()
+            e1.record(s)
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        flops = lib.tccg_probe_mfma_flops(kind) * blocks * 4 * iters * 8
+        res[name + "_tflops"] = round(flops / best / 1e12, 1)
+    res["split_bf16_ceiling_ratio"] = round(res["bf16_32x32x16_tflops"] / 6 / res["f32_32x32x2_tflops"], 3)
+    log("mfma    sustained TFLOP/s: FP32 %.1f, BF16 %.1f, FP64 %.1f; split-BF16 / FP32 ceiling %.2fx" % (
+        res["f32_32x32x2_tflops"], res["bf16_32x32x16_tflops"], res["f64_16x16x4_tflops"],
+        res["split_bf16_ceiling_ratio"]))
+    return res
+
+
 def run_log(args, log):
     """LoG (the reference's loop-over-GEMM method: single-index GEMMs batched
     over the free loops, contracted loops as accumulating launches) on the
@@ -471,6 +520,7 @@ def main():
         "grad": None,
         "ttgt": None,
         "log": None,
+        "mfma": None,
     }
     if args.only:
         forms = {args.only: forms[args.only]}
@@ -484,6 +534,8 @@ def main():
             results[key] = run_ttgt(args, log)
         elif key == "log":
             results[key] = run_log(args, log)
+        elif key == "mfma":
+            results[key] = run_mfma(args, log)
         else:
             results[key] = run_form(key, spec[0], spec[1], args, log)
     head = results.get("f32") or next(iter(results.values()))
@@ -494,11 +546,15 @@ def main():
                                                    / max(results["f32"]["rel_err_vs_fp64"], 1e-30))
     if "bw" in results:
         details["bw_hbm_roofline_frac"] = results["bw"]["kernel_gbs"] / PEAK_HBM_GBS
+    if results.get("mfma") and "split_bf16_speedup_vs_f32" in details:
+        details["split_bf16_ceiling_ratio_measured"] = results["mfma"]["split_bf16_ceiling_ratio"]
+        details["split_bf16_speedup_vs_ceiling"] = (details["split_bf16_speedup_vs_f32"]
+                                                    / results["mfma"]["split_bf16_ceiling_ratio"])
     for k in ("fp8_f32", "fp8_bf16x3", "fp8"):
         if k in results:
             details["%s_rel_err_vs_fp64" % k] = results[k].get("rel_err_vs_fp64")
     cpu = None if (args.no_cpu_baseline or args.only) else cpu_baseline(args, log)
-    if "kernel_gflops" not in head:
+    if not head or "kernel_gflops" not in head:
         # a composite form alone (--only batch / grad / ttgt): its own record only
         print(json.dumps({"metric": METRIC, "form": args.only, "forms": results}), flush=True)
         return

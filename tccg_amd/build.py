@@ -94,3 +94,24 @@ def build_library(jobs: int | None = None, force: bool = True, verbose: bool = F
     os.replace(tmp, out)
     stamp.write_text(digest + "\n")
     return out
+
+
+PROBE_NAME = "libtccg_mfma_probe.so"
+
+
+def probe_library_path() -> Path:
+    return LIB_DIR / PROBE_NAME
+
+
+def build_probe(verbose: bool = False) -> Path:
+    """The MFMA-rate probe of the measurement harness (csrc/mfma_probe.hip), a
+    library of its own: bench.py loads it, the GETT path never does."""
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    out = probe_library_path()
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-fPIC", str(CSRC / "mfma_probe.hip"), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
