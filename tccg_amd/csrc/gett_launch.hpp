@@ -307,8 +307,9 @@ constexpr int kPackedStages = 3;
 template <int P, int BM, int BN, int BK, int WM, int WN, bool PERSIST>
 int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
                          hipStream_t stream) {
-  using SH = tccg::PackedShape<P, BM, BN, BK, kPackedStages>;
-  auto kern = tccg::gett_packed_kernel<P, BM, BN, BK, WM, WN, kPackedStages, PERSIST>;
+  constexpr int ST = kPackedStages;
+  using SH = tccg::PackedShape<P, BM, BN, BK, ST>;
+  auto kern = tccg::gett_packed_kernel<P, BM, BN, BK, WM, WN, ST, PERSIST>;
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

@@ -337,24 +337,32 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   }
 
   int slot = 0;   // stage of the current step
-  bool fresh = false;
+  int fresh = 0;  // steps left whose awaited copies were issued before the last epilogue
   // one k-step: MFMAs from `cur`; with double-buffering the fragments of the
   // next step go into `nxt`, else `cur` is read from this step's stage first
   auto step = [&](FR& cur, FR& nxt) {
     // The copies waited for: with double-buffering the next step's (whose
     // fragments this step reads), else this step's.  After an epilogue its
     // memory operations also count in vmcnt.  Counts complete in issue
-    // order, and behind the awaited copies the wave always issued the later
-    // copies plus the epilogue's C-offset loads (SM + SN * NACC,
-    // unconditional; its stores may be fewer, whole-padding stores are
-    // skipped).  Waiting until at most that many operations are outstanding
+    // order.  In step j = 0 .. S-2 after an epilogue the awaited copies
+    // (step j + 1 of the tile) were issued before it, and behind them the
+    // wave issued the copies of steps j + 2 .. S-1, the epilogue's C-offset
+    // loads (SM + SN * NACC, unconditional; its stores may be fewer,
+    // whole-padding stores are skipped) and, after the epilogue, the copies
+    // of steps S .. S+j-1: at least (S - 2) * PER_WAVE + SM + SN * NACC
+    // operations.  Waiting until at most that many are outstanding
     // guarantees the copies without waiting for the C stores to drain (a
-    // count below the exact one only waits more; vmcnt stops at 63).
+    // count below the exact one only waits more; vmcnt stops at 63).  A
+    // tile shorter than S - 1 steps ends the window early: its own
+    // epilogue starts a new one.
     constexpr int kAfter = (S - 2) * SH::PER_WAVE + TS::SM + TS::SN * MM::NACC;
     constexpr int kFresh = kAfter < 63 ? kAfter : 63;
-    if (PERSIST && fresh) step_sync<kFresh>();
-    else step_sync<(S - 2) * SH::PER_WAVE>();
-    fresh = false;
+    if (PERSIST && fresh > 0) {
+      step_sync<kFresh>();
+      --fresh;
+    } else {
+      step_sync<(S - 2) * SH::PER_WAVE>();
+    }
     const int nslot = (slot + 1 == S) ? 0 : slot + 1;
     if constexpr (DB) {
       issue(slot);
@@ -405,7 +413,7 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     for (int i = 0; i < TS::SN; ++i)
 #pragma unroll
       for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
-    fresh = true;
+    fresh = S - 1;
   }
   // No copy may still be writing this workgroup's LDS when it retires.
   wait_vm_lgkm0<0>();

@@ -79,6 +79,12 @@ def lds_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
     return 2 * _PLANES[prec] * (BM + BN) * (BK + pad) * eb
 
 
+def packed_ring_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
+    """LDS of the packed kernel: a ring of 3 k-step stages (kPackedStages in
+    gett_launch.hpp)."""
+    return 3 * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec]
+
+
 class UnsupportedContraction(ValueError):
     """The contraction cannot be expressed as a GETT (e.g. batch indices)."""
 
@@ -409,10 +415,10 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
                     ident_m, ident_n, modeA, modeB, groupM, packed=True)
     if not persistent:
         esz = 8 if prec == F64 else 4
-        ring = 3 * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec]   # LDS of the packed kernel
+        ring = packed_ring_bytes(prec, BM, BN, BK)
         plan.crun = c_runs(offCm, offCn, BM, BN, esz, ring // ((BM + 16 // esz) * esz))
     if persistent:
-        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // lds_bytes(prec, BM, BN, BK)))
+        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // packed_ring_bytes(prec, BM, BN, BK)))
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
         if plan.grid >= plan.ntiles():
             plan.grid = 0
@@ -621,7 +627,8 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
         t_valu = plan.padded_flops() / (0.5 * hw["valu"][plan.prec])
         t = max(t_hbm, t_valu)
         return t, (0.0 if plan.modeA == FREE4 else t)
-    lds = lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK)
+    lds = (packed_ring_bytes(plan.prec, plan.BM, plan.BN, plan.BK) if plan.packed
+           else lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK))
     per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
     slots = hw["cus"] * per_cu
     ntiles = plan.ntiles()

@@ -77,6 +77,24 @@ def test_packed_persistent_walk(prec):
     check(c, prec, seed=11, plan=plan)
 
 
+@pytest.mark.parametrize("steps", [1, 2, 3, 5])
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3, FP8])
+def test_packed_persistent_every_tile(prec, steps):
+    """The persistent walk of every packed tile
+    with a few k-steps per tile (`steps` pack blocks of k; 16-deep tiles get
+    twice as many): tiles shorter than the ring, where the waits after one
+    epilogue run into the next, and longer ones."""
+    for tile, (BM, BN, BK) in enumerate(PACKED_TILES[prec]):
+        kb = max(BK, 32)    # k is padded to whole 32-deep pack blocks
+        c = Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"],
+                                    {"a": 2 * BM + 17, "b": 3 * BN + 5, "k": steps * kb - 3})
+        plan = build_packed_plan(c.problem, prec, tile)
+        assert plan.ntiles() > 5 and plan.Kp == steps * kb, (plan.ntiles(), plan.Kp)
+        plan.grid = 5
+        print("persistent", prec, steps, plan.describe(), flush=True)
+        check(c, prec, seed=tile + 10 * steps, plan=plan, tol=_tol(prec))
+
+
 # (precision, packed tile id, unpacked tile id) with the same (BM, BN, BK)
 SAME_SHAPE = [(F32, 0, 0), (F32, 2, 5), (F64, 0, 0), (BF16X3, 1, 0), (BF16X3, 0, 3), (BF16, 0, 0), (FP8, 0, 0)]
 
