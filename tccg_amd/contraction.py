@@ -41,6 +41,17 @@ class Contraction:
         # Batch indices are rejected before fusion: fusing a run that contains
         # one would leave it in a single tensor (the reference fuses first,
         # tccg/tccg.py:104, and only then checks, tccg/gett.py:42-45).
+        for T in (A, B, C):
+            for i in T.indices:
+                if i.size < 1:
+                    raise UnsupportedContraction("index %s of %s has extent %d: extents must be positive"
+                                                 % (i.label, T.label, i.size))
+            # Kernels add the uint32 per-set offsets of an element in 32 bits:
+            # every element offset, hence every such sum, must stay below the
+            # INVALID sentinel 2^32 - 1.
+            if T.size() >= 0xFFFFFFFF:
+                raise UnsupportedContraction("tensor %s has %d elements; at most 2^32 - 2 are addressable"
+                                             % (T.label, T.size()))
         loop = get_loop_indices(A, B, C)
         if loop:
             raise UnsupportedContraction("loop indices are not yet supported by GETT: %s"

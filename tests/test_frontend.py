@@ -153,3 +153,23 @@ def test_contraction_str():
     C, A, B, al, be = parse_tccg(EXAMPLE)
     assert contraction_str(C, A, B, al, be) == \
         "C((a,24),(b,24),(i,24),(j,24)) = 1.00 * A((i,24),(m,24),(a,24)) B((m,24),(j,24),(b,24)) + 0.00 * C((a,24),(b,24),(i,24),(j,24))"
+
+
+@pytest.mark.parametrize("sizes", [{"a": 0, "b": 4, "k": 3}, {"a": 4, "b": 4, "k": 0}])
+def test_zero_extent_is_rejected(sizes):
+    """An empty index set is rejected up front with a message naming the
+    index (the generator has no empty-contraction path)."""
+    with pytest.raises(UnsupportedContraction, match="extent 0"):
+        Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"], sizes)
+    text = "C[a,b] = A[a,k] * B[k,b]\n" + "".join("%s = %d\n" % kv for kv in sizes.items())
+    with pytest.raises(UnsupportedContraction, match="extent 0"):
+        Contraction.from_tccg(text)
+
+
+def test_tensor_beyond_32bit_offsets_is_rejected():
+    """Element offsets are 32-bit: a tensor of 2^32 - 1 or more elements is
+    rejected (no tables are built); one just below the limit is accepted."""
+    with pytest.raises(UnsupportedContraction, match="2\\^32 - 2"):
+        Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"], {"a": 65536, "b": 2, "k": 65536})
+    c = Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"], {"a": 65535, "b": 2, "k": 65535})
+    assert c.A.size() == 65535 * 65535
