@@ -317,6 +317,10 @@ int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG
     if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
     attr_set = true;
   }
+  if (PERSIST) {
+    const int cap = resident_limit(reinterpret_cast<const void*>(kern), SH::LDS_BYTES);
+    if (cap > 0 && grid > cap) grid = cap;
+  }
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), SH::LDS_BYTES, stream, prm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
