@@ -83,6 +83,8 @@ typedef struct tccg_gett_plan {
   int32_t grid;        /* 0: one workgroup per C tile; > 0: that many       */
                        /* persistent workgroups walk the tiles, prefetching */
                        /* the next tile's first k-step during the epilogue  */
+                       /* (at most the workgroups resident at once: a      */
+                       /* larger grid is clamped to that)                   */
   const float* amax;   /* TCCG_FP8 only: device float[2] = max|A|, max|B|   */
   int32_t walk;        /* 0: as given by grid; 2: streaming walk over grid  */
                        /* workgroups for single-k-step plans (Kp == BK):    */
