@@ -360,7 +360,10 @@ def run_mfma(args, log):
     lib.tccg_probe_mfma_round.argtypes = [ctypes.c_int]
     s = torch.cuda.current_stream()
     res = {}
-    for kind, name, iters in ((0, "f32_32x32x2", 1000), (1, "bf16_32x32x16", 2000), (2, "f64_16x16x4", 1000)):
+    # (kind, name, iterations, MFMAs per wave and iteration)
+    for kind, name, iters, per_iter in ((0, "f32_32x32x2", 1000, 8), (1, "bf16_32x32x16", 2000, 8),
+                                        (2, "f64_16x16x4", 1000, 8), (3, "bf16x3_gett_pattern", 300, 54),
+                                        (4, "f32_gett_pattern", 120, 72)):
         rnd = lib.tccg_probe_mfma_round(kind)
         if rnd <= 0:
             raise RuntimeError("mfma probe: no resident workgroups")
@@ -381,12 +384,17 @@ def run_mfma(args, log):
             e1.record(s)
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1) * 1e-3)
-        flops = lib.tccg_probe_mfma_flops(kind) * blocks * 4 * iters * 8
+        flops = lib.tccg_probe_mfma_flops(kind) * blocks * 4 * iters * per_iter
         res[name + "_tflops"] = round(flops / best / 1e12, 1)
     res["split_bf16_ceiling_ratio"] = round(res["bf16_32x32x16_tflops"] / 6 / res["f32_32x32x2_tflops"], 3)
+    res["split_bf16_ceiling_ratio_gett_pattern"] = round(
+        res["bf16x3_gett_pattern_tflops"] / 6 / res["f32_gett_pattern_tflops"], 3)
     log("mfma    sustained TFLOP/s: FP32 %.1f, BF16 %.1f, FP64 %.1f; split-BF16 / FP32 ceiling %.2fx" % (
         res["f32_32x32x2_tflops"], res["bf16_32x32x16_tflops"], res["f64_16x16x4_tflops"],
         res["split_bf16_ceiling_ratio"]))
+    log("mfma    in the GETT's operand pattern (3 x 3 fragments, 96 x 96 per wave): FP32 %.1f, split-BF16 "
+        "products %.1f TFLOP/s; ceiling %.2fx" % (res["f32_gett_pattern_tflops"], res["bf16x3_gett_pattern_tflops"],
+                                                 res["split_bf16_ceiling_ratio_gett_pattern"]))
     return res
 
 
