@@ -64,13 +64,9 @@ int tccg_gett_batch_f32(const tccg_batch_problem* problems, const tccg_batch_wor
   if (nwork <= 0) return fail(1, "empty work list");
   if (grid <= 0) return fail(1, "grid must be positive");
   if (grid > nwork) grid = nwork;
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(gett_batch_kernel_f32),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kBatchLdsBytes);
-    if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    attr = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  if (int rc = ensure_dynamic_lds(reinterpret_cast<const void*>(gett_batch_kernel_f32), kBatchLdsBytes, lds_set))
+    return rc;
   hipLaunchKernelGGL(gett_batch_kernel_f32, dim3(grid), dim3(256), kBatchLdsBytes, (hipStream_t)stream, problems,
                      work, nwork);
   hipError_t e = hipGetLastError();

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <atomic>
 #include <string>
 
 #include "gett_core.hpp"
@@ -15,6 +17,21 @@ namespace tccg_lib {
 int fail(int code, const std::string& msg);
 
 struct TileCfg { int BM, BN, BK, WM, WN; };
+
+// Raises `kern`'s dynamic-LDS limit to `bytes` on the current device once:
+// the attribute is set per device, so `done` holds one bit per device id
+// (ids of 64 and above set it on every call).  Safe from several host threads.
+inline int ensure_dynamic_lds(const void* kern, int bytes, std::atomic<uint64_t>& done) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail((int)e, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  const uint64_t bit = dev < 64 ? (uint64_t{1} << dev) : 0;
+  if (bit && (done.load(std::memory_order_acquire) & bit)) return 0;
+  e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  done.fetch_or(bit, std::memory_order_release);
+  return 0;
+}
 
 // Tile configurations per precision; ids index these arrays.
 constexpr TileCfg kF32Tiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 256, 16, 2, 2},
@@ -115,14 +132,9 @@ template <int P, int BM, int BN, int BK, int WM, int WN, int MA, int MB, int WAL
 int launch_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid, hipStream_t stream) {
   using S = tccg::TileShape<P, BM, BN, BK, WM, WN>;
   auto kern = tccg::gett_kernel<P, BM, BN, BK, WM, WN, MA, MB, WALK>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       WALK == 2 ? kMaxLds : S::LDS_BYTES);
-    if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  if (int rc = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), WALK == 2 ? kMaxLds : S::LDS_BYTES, lds_set))
+    return rc;
   // A single k-step never touches the second LDS stage.
   int lds = prm.kTiles == 1 ? S::LDS_BYTES / 2 : S::LDS_BYTES;
   if (WALK == 2) lds = (int)tccg::stream_lds_bytes<P, BM, BN, BK, WM, WN>(prm.tilesN);
@@ -310,13 +322,8 @@ int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG
   constexpr int ST = kPackedStages;
   using SH = tccg::PackedShape<P, BM, BN, BK, ST>;
   auto kern = tccg::gett_packed_kernel<P, BM, BN, BK, WM, WN, ST, PERSIST>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, SH::LDS_BYTES);
-    if (e != hipSuccess) return fail((int)e, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  if (int rc = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), SH::LDS_BYTES, lds_set)) return rc;
   if (PERSIST) {
     const int cap = resident_limit(reinterpret_cast<const void*>(kern), SH::LDS_BYTES);
     if (cap > 0 && grid > cap) grid = cap;
