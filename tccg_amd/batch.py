@@ -84,7 +84,8 @@ class GettBatch:
         from .runtime import load_library
         from .runtime import DevicePlan
         lib = load_library()
-        dev = device or As[0].device
+        from .runtime import cuda_device
+        dev = cuda_device(torch, device if device is not None else As[0].device)
         # problems on the packed path: one shared workspace (they run one after the other)
         large = []
         if self.large:
@@ -115,9 +116,9 @@ class GettBatch:
             i = small[j]
             p, c, A, B, C = self.plans[i], self.contractions[i], As[i], Bs[i], Cs[i]
             for name, t, T in (("A", A, c.A0), ("B", B, c.B0), ("C", C, c.C0)):
-                if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size():
-                    raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements"
-                                     % (i, name, T.size()))
+                if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device != dev:
+                    raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements on %s"
+                                     % (i, name, T.size(), dev))
             opA, opB = (B, A) if p.problem.swapped else (A, B)
             q = probs[j]
             q.A, q.B, q.C = opA.data_ptr(), opB.data_ptr(), C.data_ptr()
@@ -149,14 +150,17 @@ class GettBatch:
         b = self._bound
         if b is None:
             raise RuntimeError("bind() the batch to tensors first")
-        s = stream if stream is not None else torch.cuda.current_stream()
+        dev = b["work"].device
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
         As, Bs, Cs = b["tensors"]
         for dp, i in b["large"]:   # one after the other: they share the workspace
             dp.run(As[i], Bs[i], Cs[i], self.alphas[i], self.betas[i], stream=s)
         if b["nwork"] == 0:
             return
-        rc = b["lib"].tccg_gett_batch_f32(ctypes.c_void_p(b["probs"].data_ptr()), ctypes.c_void_p(b["work"].data_ptr()),
-                                          b["nwork"], b["grid"], ctypes.c_void_p(s.cuda_stream))
+        with torch.cuda.device(dev):
+            rc = b["lib"].tccg_gett_batch_f32(ctypes.c_void_p(b["probs"].data_ptr()),
+                                              ctypes.c_void_p(b["work"].data_ptr()), b["nwork"], b["grid"],
+                                              ctypes.c_void_p(s.cuda_stream))
         if rc != 0:
             raise GettRuntimeError("batched GETT failed: %s" % b["lib"].tccg_last_error().decode())
 

@@ -216,8 +216,10 @@ class DeviceLoG:
         alpha = contraction.alpha if alpha is None else alpha
         beta = contraction.beta if beta is None else beta
         for name, t, T in (("A", A, contraction.A0), ("B", B, contraction.B0), ("C", C, contraction.C0)):
-            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device.type != "cuda":
-                raise ValueError("LoG: %s must be a contiguous float32 GPU tensor of %d elements" % (name, T.size()))
+            if (t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device.type != "cuda"
+                    or t.device != C.device):
+                raise ValueError("LoG: %s must be a contiguous float32 tensor of %d elements on C's GPU"
+                                 % (name, T.size()))
         opA, opB = (B, A) if pb.swapped else (A, B)
         lay = log_layout(pb, cand, shape)
         self.layout = lay
@@ -252,12 +254,15 @@ class DeviceLoG:
     def launch(self, stream=None) -> None:
         import torch
         from .runtime import GettRuntimeError
-        s = stream if stream is not None else torch.cuda.current_stream()
-        for probs in self.launches:
-            rc = self.lib.tccg_gett_batch_f32(ctypes.c_void_p(probs.data_ptr()), ctypes.c_void_p(self.work.data_ptr()),
-                                              self.nwork, self.grid, ctypes.c_void_p(s.cuda_stream))
-            if rc != 0:
-                raise GettRuntimeError("LoG launch failed: %s" % self.lib.tccg_last_error().decode())
+        dev = self.tables.device
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        with torch.cuda.device(dev):
+            for probs in self.launches:
+                rc = self.lib.tccg_gett_batch_f32(ctypes.c_void_p(probs.data_ptr()),
+                                                  ctypes.c_void_p(self.work.data_ptr()), self.nwork, self.grid,
+                                                  ctypes.c_void_p(s.cuda_stream))
+                if rc != 0:
+                    raise GettRuntimeError("LoG launch failed: %s" % self.lib.tccg_last_error().decode())
 
     def run(self, A=None, B=None, C=None, alpha=None, beta=None, stream=None) -> None:
         """Launch on the bound tensors (the arguments exist for the timing

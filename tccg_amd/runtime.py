@@ -138,6 +138,31 @@ def _torch():
     return torch
 
 
+def check_tensor(name: str, t, dtype, numel: int, device) -> None:
+    """A kernel argument must be a contiguous tensor of `dtype` with `numel`
+    elements on `device` (the device the plan's tables live on): anything
+    else would be read or written out of bounds or across devices."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch.Tensor" % name)
+    if t.device.type != "cuda" or t.device.index != device.index:
+        raise GettRuntimeError("%s is on %s; the plan's tables are on %s" % (name, t.device, device))
+    if t.dtype != dtype:
+        raise TypeError("%s has dtype %s, expected %s" % (name, t.dtype, dtype))
+    if not t.is_contiguous():
+        raise GettRuntimeError("%s must be contiguous (TCCG layout, leftmost index stride-one)" % name)
+    if t.numel() != numel:
+        raise GettRuntimeError("%s has %d elements, %d expected" % (name, t.numel(), numel))
+
+
+def cuda_device(torch, device):
+    """torch.device of `device` (default: the current one) with its index set."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if dev.type != "cuda":
+        raise GettRuntimeError("the gfx950 GETT path runs on GPU devices, not %s" % dev)
+    return dev if dev.index is not None else torch.device("cuda", torch.cuda.current_device())
+
+
 def _check_cuda():
     torch = _torch()
     if not torch.cuda.is_available():
@@ -154,7 +179,7 @@ class DevicePlan:
         several packed plans one after the other can share one)."""
         torch = _check_cuda()
         self.plan = plan
-        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        dev = cuda_device(torch, device)
         self.device = dev
 
         def up(a: np.ndarray, dt):
@@ -193,24 +218,20 @@ class DevicePlan:
         torch = _torch()
         want = torch.float64 if self.plan.prec == F64 else torch.float32
         for name, t, T in (("A", A, pb.A), ("B", B, pb.B), ("C", C, pb.C)):
-            if not isinstance(t, torch.Tensor):
-                raise TypeError("%s must be a torch.Tensor" % name)
-            if t.device.type != "cuda":
-                raise GettRuntimeError("%s must be a GPU tensor" % name)
-            if t.dtype != want:
-                raise TypeError("%s has dtype %s, expected %s" % (name, t.dtype, want))
-            if not t.is_contiguous():
-                raise GettRuntimeError("%s must be contiguous (TCCG layout, leftmost index stride-one)" % name)
-            if t.numel() != T.size():
-                raise GettRuntimeError("%s has %d elements, the contraction needs %d" % (name, t.numel(), T.size()))
+            check_tensor(name, t, want, T.size(), self.device)
         return (B, A) if pb.swapped else (A, B)
 
     def run(self, A, B, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         """C <- alpha * contract(A, B) + beta * C, asynchronously on `stream`
         (default: torch's current stream)."""
         torch = _torch()
-        lib = load_library()
         opA, opB = self._operands(A, B, C)
+        with torch.cuda.device(self.device):
+            self._launch(opA, opB, C, alpha, beta, stream)
+
+    def _launch(self, opA, opB, C, alpha, beta, stream) -> None:
+        torch = _torch()
+        lib = load_library()
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         sptr = ctypes.c_void_p(s.cuda_stream)
         prec = self.plan.prec
@@ -238,10 +259,13 @@ class DevicePlan:
         torch = _torch()
         if not self.plan.packed:
             raise GettRuntimeError("run_prepacked() needs a packed plan")
+        check_tensor("C", C, torch.float64 if self.plan.prec == F64 else torch.float32, self.plan.problem.C.size(),
+                     self.device)
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
-        rc = load_library().tccg_gett_prepacked(self.plan.prec, C.data_ptr(), alpha, beta, ctypes.byref(self.cplan),
-                                                self.workspace.data_ptr(), self.workspace.numel(),
-                                                ctypes.c_void_p(s.cuda_stream))
+        with torch.cuda.device(self.device):
+            rc = load_library().tccg_gett_prepacked(self.plan.prec, C.data_ptr(), alpha, beta,
+                                                    ctypes.byref(self.cplan), self.workspace.data_ptr(),
+                                                    self.workspace.numel(), ctypes.c_void_p(s.cuda_stream))
         if rc != 0:
             raise GettRuntimeError("GETT launch failed: %s" % last_error())
 

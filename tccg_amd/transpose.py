@@ -210,10 +210,10 @@ class DeviceTranspose:
     tccg_transpose_f32 / tccg_transpose_f64."""
 
     def __init__(self, plan: TransposePlan, device=None):
-        from .runtime import _check_cuda
+        from .runtime import _check_cuda, cuda_device
         torch = _check_cuda()
         self.plan = plan
-        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        dev = cuda_device(torch, device)
         self.device = dev
         up = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32)).to(dev)
         self.tables = [up(a) for a in (plan.inX, plan.outX, plan.inY, plan.outY, plan.inZ, plan.outZ)]
@@ -224,20 +224,19 @@ class DeviceTranspose:
         self.cplan = c
 
     def run(self, inp, out, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
-        from .runtime import GettRuntimeError, last_error, load_library, _torch
+        from .runtime import GettRuntimeError, check_tensor, last_error, load_library, _torch
         torch = _torch()
-        n = self.plan.elements()
-        for name, t in (("input", inp), ("output", out)):
-            if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or not t.is_contiguous():
-                raise GettRuntimeError("%s must be a contiguous GPU tensor" % name)
-            if t.numel() != n:
-                raise GettRuntimeError("%s has %d elements, the transposition needs %d" % (name, t.numel(), n))
-        if inp.dtype != out.dtype or inp.dtype not in (torch.float32, torch.float64):
+        if not isinstance(inp, torch.Tensor) or inp.dtype not in (torch.float32, torch.float64):
             raise TypeError("transpose: float32 or float64 tensors of one dtype")
+        n = self.plan.elements()
+        check_tensor("input", inp, inp.dtype, n, self.device)
+        check_tensor("output", out, inp.dtype, n, self.device)
         lib = load_library()
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         fn = lib.tccg_transpose_f64 if inp.dtype == torch.float64 else lib.tccg_transpose_f32
-        rc = fn(inp.data_ptr(), out.data_ptr(), alpha, beta, ctypes.byref(self.cplan), ctypes.c_void_p(s.cuda_stream))
+        with torch.cuda.device(self.device):
+            rc = fn(inp.data_ptr(), out.data_ptr(), alpha, beta, ctypes.byref(self.cplan),
+                    ctypes.c_void_p(s.cuda_stream))
         if rc != 0:
             raise GettRuntimeError("transpose launch failed: %s" % last_error())
 

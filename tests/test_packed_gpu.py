@@ -172,3 +172,24 @@ def test_prepacked_reuses_packed_operands(prec):
     dp.run_prepacked(tC)
     torch.cuda.synchronize()
     assert torch.equal(first, tC)
+
+
+def test_bad_tensors_are_rejected_before_launch():
+    """Host tensors, wrong element counts and wrong dtypes raise before any
+    kernel is launched, for run() and run_prepacked() alike."""
+    import torch
+
+    from tccg_amd.runtime import GettRuntimeError
+    c = Contraction.from_tccg(scaled("abcd-aebf-dfce", 0.2))
+    dp = c.device_plan(plan=build_packed_plan(c.problem, F32, 0))
+    A, B, C = (torch.zeros(s, dtype=torch.float32, device="cuda") for s in c.shapes())
+    with pytest.raises(GettRuntimeError, match="tables are on"):
+        dp.run(A.cpu(), B, C)
+    with pytest.raises(GettRuntimeError, match="elements"):
+        dp.run(A, B, C.reshape(-1)[:-1])
+    with pytest.raises(TypeError, match="dtype"):
+        dp.run(A, B.double(), C)
+    dp.run(A, B, C)
+    torch.cuda.synchronize()
+    with pytest.raises(GettRuntimeError, match="elements"):
+        dp.run_prepacked(C.reshape(-1)[:-1])
