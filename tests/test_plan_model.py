@@ -1,0 +1,70 @@
+"""CPU model of the table-driven addressing every GETT kernel shares, over
+random contractions and every candidate plan (gather, blocked, persistent,
+streaming, packed, vector): element (m, k) of the internal A operand is read
+at offAm[m] + offAk[k], B's (n, k) at offBn[n] + offBk[k], and C's (m, n)
+written at offCm[m] + offCn[n]; positions marked INVALID are read as zero and
+never written.  The product computed through the tables must equal the FP64
+einsum oracle, every valid access must be in bounds, the validity of A's and
+C's m tables (B's and C's n tables, A's and B's k tables) must agree, and every
+element of C must be written exactly once."""
+import numpy as np
+import pytest
+
+from oracle.einsum_ref import contract_ref, rel_frobenius
+from tccg_amd import Contraction
+from tccg_amd.plan import F32, F64, INVALID, candidate_plans
+
+INV = np.int64(INVALID)
+
+
+def random_contraction(rng):
+    labels = iter("abcdefghij")
+    m = [next(labels) for _ in range(rng.integers(1, 3))]
+    n = [next(labels) for _ in range(rng.integers(1, 3))]
+    k = [next(labels) for _ in range(rng.integers(1, 3))]
+    sizes = {l: int(rng.integers(1, 10)) for l in m + n + k}
+    perm = lambda xs: [xs[i] for i in rng.permutation(len(xs))]
+    return Contraction.from_labels(perm(m + n), perm(m + k), perm(n + k), sizes)
+
+
+def emulate(plan, A, B, C, alpha, beta):
+    pb = plan.problem
+    opA, opB = (B, A) if pb.swapped else (A, B)
+    a, b, c = opA.reshape(-1), opB.reshape(-1), C.reshape(-1).copy()
+    t = {key: getattr(plan, key).astype(np.int64) for key in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn")}
+    vm, vn, vk = t["offAm"] != INV, t["offBn"] != INV, t["offAk"] != INV
+    assert np.array_equal(vm, t["offCm"] != INV) and np.array_equal(vn, t["offCn"] != INV)
+    assert np.array_equal(vk, t["offBk"] != INV)
+    am, ak, bn, bk = t["offAm"][vm], t["offAk"][vk], t["offBn"][vn], t["offBk"][vk]
+    cm, cn = t["offCm"][vm], t["offCn"][vn]
+    # bounds of every valid access
+    assert am.max() + ak.max() < a.size and bn.max() + bk.max() < b.size
+    assert cm.max() + cn.max() < c.size
+    # every element of C written exactly once
+    idx = (cm[:, None] + cn[None, :]).reshape(-1)
+    assert idx.size == c.size and np.unique(idx).size == c.size
+    Am = a[am[:, None] + ak[None, :]]
+    Bm = b[bn[:, None] + bk[None, :]]
+    out = alpha * (Am @ Bm.T)
+    cidx = cm[:, None] + cn[None, :]
+    c[cidx] = out + beta * c[cidx]
+    return c.reshape(C.shape)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_every_plan_addresses_the_contraction(seed):
+    rng = np.random.default_rng(seed)
+    c = random_contraction(rng)
+    sa, sb, sc = c.shapes()
+    A, B, C = rng.uniform(-1, 1, sa), rng.uniform(-1, 1, sb), rng.uniform(-1, 1, sc)
+    labels = lambda T: [i.label for i in T.indices]
+    ref = contract_ref(labels(c.C0), labels(c.A0), labels(c.B0), A, B, C, 1.25, -0.5)
+    prec = F64 if seed % 2 else F32
+    plans = candidate_plans(c.problem, prec)
+    assert plans
+    kinds = set()
+    for plan in plans:
+        kinds.add((plan.packed, bool(plan.grid), plan.walk))
+        out = emulate(plan, A, B, C, 1.25, -0.5)
+        assert rel_frobenius(out, ref) < 1e-13, (str(c), plan.describe())
+    assert any(p for p, _, _ in kinds) and any(not p for p, _, _ in kinds), kinds
