@@ -117,7 +117,10 @@ pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __
   static_assert(BK % 8 == 0 && (BK <= TK ? TK % BK == 0 : BK % TK == 0), "pack block vs k-step");
   __shared__ TG tile[TF][TK + 1];
 
-  const int f0 = blockIdx.x * TF, k0 = blockIdx.y * TK;
+  // consecutive blocks along the free set run on one XCD: neighbours that
+  // share the partial cache lines of a stride-one run meet in the same L2
+  const int lb = xcd_chunked(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int f0 = (lb % gridDim.x) * TF, k0 = (lb / gridDim.x) * TK;
   const int t = threadIdx.x;
   if (kfast) {
     const int k = t & (TK - 1);
