@@ -37,6 +37,16 @@ struct TransposeParams {
   T alpha, beta;
 };
 
+// Workgroup ids are dealt to the 8 XCDs (each with its own L2) in turn; this
+// maps them so that XCD x runs one contiguous range of tiles.  Neighbouring
+// tiles, which share the partial cache lines at the ends of their rows, then
+// meet in one L2.
+__device__ inline int transpose_tile_id(int bid, int nblocks) {
+  constexpr int NX = 8;
+  const int q = nblocks / NX, r = nblocks % NX, x = bid % NX, l = bid / NX;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
 // VEC consecutive elements (16 bytes when VEC > 1) as one access.
 template <class T, int VEC> struct RowVec {
   typedef T type __attribute__((ext_vector_type(VEC)));
@@ -57,7 +67,8 @@ template <class T> struct VecLane<T, 1> {
 // extents of X and Y are multiples of VEC and the tensors 16-byte aligned,
 // so groups of VEC x (or y) positions are contiguous, aligned and valid or
 // invalid as a whole.  Loads: the lanes along x; stores: along y.
-// Grid: (Xp / TX) * (Yp / TY) * Z workgroups, x-tile fastest.
+// Grid: (Xp / TX) * (Yp / TY) * Z workgroups, x-tile fastest in the tile
+// order of transpose_tile_id().
 template <class T, int TX, int TY, int VEC>
 __global__ void __launch_bounds__(256)
 transpose_tile(TransposeParams<T> p) {
@@ -71,7 +82,7 @@ transpose_tile(TransposeParams<T> p) {
   const int t = threadIdx.x;
   const int tilesX = p.Xp / TX, tilesY = p.Yp / TY;
   const int per_z = tilesX * tilesY;
-  const int b = blockIdx.x;
+  const int b = transpose_tile_id(blockIdx.x, gridDim.x);
   const int z = b / per_z, r = b - z * per_z;
   const int xt = r % tilesX, yt = r / tilesX;
   const size_t iz = p.inZ[z], oz = p.outZ[z];
