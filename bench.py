@@ -562,8 +562,8 @@ def main():
         if k in results:
             details["%s_rel_err_vs_fp64" % k] = results[k].get("rel_err_vs_fp64")
     cpu = None if (args.no_cpu_baseline or args.only) else cpu_baseline(args, log)
-    if not head or "kernel_gflops" not in head:
-        # a composite form alone (--only batch / grad / ttgt): its own record only
+    if not head or "kernel_gflops" not in head or "prec" not in head:
+        # a composite form alone (--only batch / grad / ttgt / log / mfma): its own record only
         print(json.dumps({"metric": METRIC, "form": args.only, "forms": results}), flush=True)
         return
     traffic = profiled_traffic(head["workload"], head["prec"], head["plan"])
