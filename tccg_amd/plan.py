@@ -48,7 +48,7 @@ TILES: Dict[int, List[Tuple[int, int, int]]] = {
 # Tile shapes of the packed path (operands converted once into tile-major
 # planes by the pack kernel); must match csrc/gett_launch.hpp's packed tables.
 PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
-    F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16)],
+    F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16), (128, 64, 16), (64, 128, 16)],
     F64: [(128, 128, 16), (64, 64, 16)],
     BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 192, 16)],
     FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128), (192, 192, 128)],
