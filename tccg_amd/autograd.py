@@ -61,11 +61,13 @@ class GettFunction:
                     adjA, adjB = adjoints(c)
                     dC = dC.contiguous()
                     dA = dB = None
+                    # gradients are written in the TCCG layout (contiguous),
+                    # whatever the strides of the forward's inputs were
                     if ctx.needs_input_grad[0]:
-                        dA = torch.empty_like(A)
+                        dA = torch.empty(A.shape, dtype=A.dtype, device=A.device)
                         adjA(dC, B.contiguous(), dA, alpha=c.alpha, beta=0.0, prec=prec)
                     if ctx.needs_input_grad[1]:
-                        dB = torch.empty_like(B)
+                        dB = torch.empty(B.shape, dtype=B.dtype, device=B.device)
                         adjB(A.contiguous(), dC, dB, alpha=c.alpha, beta=0.0, prec=prec)
                     return dA, dB, None, None
 

@@ -38,7 +38,7 @@ from pathlib import Path
 
 from .frontend import TccgSyntaxError, contraction_str, fuse_indices, read_tccg_file
 from .plan import (BF16X3, F32, F64, FP8, PREC_NAMES, UnsupportedContraction, candidate_plans, build_plan,
-                   estimate_seconds, plan_from_kind, plan_kind)
+                   estimate_seconds, plan_from_kind, plan_kind, tile_shape_of, tile_tables_digest)
 
 OKGREEN = "\033[92m"
 FAIL = "\033[91m"
@@ -184,10 +184,13 @@ def main(argv=None) -> int:
     if gpu:
         from .tuning_db import TuningDB
         db = TuningDB(args.database)
-        device = "gfx950"
+        device = "gfx950/" + tile_tables_digest()
         hit = None if args.ignoreDatabase else db.fastest(contraction, device, PREC_NAMES[prec], contraction.beta)
+        if hit is not None and tile_shape_of(prec, hit[1], hit[3]) != hit[5]:
+            print(WARNING + "[database] entry %s names a tile of another shape; re-tuning" % hit[0] + ENDC)
+            hit = None
         if hit is not None:
-            name, tile, blocked, kind, gf = hit
+            name, tile, blocked, kind, gf, _shape = hit
             chosen = plan_from_kind(pb, prec, tile, blocked, kind)
             best_gflops = gf
             print(OKGREEN + "[database] reusing %s (%.2f GFLOPS)" % (name, gf) + ENDC)
@@ -354,7 +357,7 @@ def _time_candidates(args, contraction, prec, plans, db, device):
         gf = contraction.flops() / t / 1e9
         est = contraction.flops() / estimate_seconds(p, contraction.beta) / 1e9
         print("%s: %.2f GFLOPS // estimated: %.2f GFLOPS" % (name, gf, est))
-        db.insert_gett(mid, name, p.tile, not p.lin_m.is_plain(), gf, est, plan_kind(p))
+        db.insert_gett(mid, name, p.tile, not p.lin_m.is_plain(), gf, est, plan_kind(p), (p.BM, p.BN, p.BK))
         if gf > best[0]:
             best = (gf, p)
     if failed:

@@ -460,11 +460,24 @@ struct TileShape {
   static_assert(BK % TR::KF == 0, "k tile");
 };
 
+// s_waitcnt vmcnt(N) lgkmcnt(0) (expcnt left at its maximum).  The builtin
+// has side effects: the compiler keeps it between the memory operations
+// issued before and after it.
+template <int N>
+__device__ inline void wait_vm_lgkm0() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xf) | (0x7 << 4) | (0x0 << 8) | ((N >> 4) << 14));
+}
+
 // Epilogue of one tile: alpha/beta update of C along its stride-one index.
 // The C offsets of this lane's rows and columns are gathered before any
 // store; cmT / cnT point at the offsets of the tile's first m / n position
-// (the global offset tables, or copies of them in LDS).
-template <int P, int BM, int BN, int BK, int WM, int WN>
+// (the global offset tables, or copies of them in LDS).  DRAIN: wait until
+// every vector-memory operation this wave issued before the first store has
+// completed (the offset loads and whatever was in flight before them), so
+// that no later wait of this wave has to count them; the C stores are not
+// waited for.
+template <int P, int BM, int BN, int BK, int WM, int WN, bool DRAIN = false>
 __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::TG>& p,
                                        typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
                                                                   [TileShape<P, BM, BN, BK, WM, WN>::SM],
@@ -490,6 +503,7 @@ __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::
       MM::coord(lane, r, n_l, m_l);
       cn[i][r] = cnT[wn * S::WTN + i * TR::SUB + n_l];
     }
+  if constexpr (DRAIN) wait_vm_lgkm0<0>();
   // Row by row: the SM column blocks of one C row (consecutive m, i.e.
   // consecutive addresses along C's stride-one index) are stored back to
   // back.  Common case first: beta == 0 and every position of the wave's
@@ -535,12 +549,12 @@ __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::
   }
 }
 
-template <int P, int BM, int BN, int BK, int WM, int WN>
+template <int P, int BM, int BN, int BK, int WM, int WN, bool DRAIN = false>
 __device__ inline void store_tile(const GettParams<typename PrecTraits<P>::TG>& p,
                                   typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
                                                              [TileShape<P, BM, BN, BK, WM, WN>::SM],
                                   int m0, int n0, int wm, int wn, int lane, float descale) {
-  store_tile_from<P, BM, BN, BK, WM, WN>(p, acc, p.offCm + m0, p.offCn + n0, wm, wn, lane, descale);
+  store_tile_from<P, BM, BN, BK, WM, WN, DRAIN>(p, acc, p.offCm + m0, p.offCn + n0, wm, wn, lane, descale);
 }
 
 // One BM x BN tile (tm, tn) of C over the whole k range, computed by the 256

@@ -45,13 +45,6 @@ __host__ __device__ constexpr int chunk_pos(int F, int r, int c) {
   return (c >> 1) * (F * KS) + r * KS + (((c & 1) ^ swz2(r)) * VE);
 }
 
-// s_waitcnt vmcnt(N) lgkmcnt(0) (expcnt left at its maximum).
-template <int N>
-__device__ inline void wait_vm_lgkm0() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 0xf) | (0x7 << 4) | (0x0 << 8) | ((N >> 4) << 14));
-}
-
 // The step boundary: s_waitcnt vmcnt(N) lgkmcnt(0), then a workgroup barrier
 // without a memory fence.  The scheduling barriers keep every instruction
 // (MFMAs included) on its side, so the wait follows all of the previous
@@ -342,23 +335,19 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   // next step go into `nxt`, else `cur` is read from this step's stage first
   auto step = [&](FR& cur, FR& nxt) {
     // The copies waited for: with double-buffering the next step's (whose
-    // fragments this step reads), else this step's.  After an epilogue its
-    // memory operations also count in vmcnt.  Counts complete in issue
-    // order.  In step j = 0 .. S-2 after an epilogue the awaited copies
-    // (step j + 1 of the tile) were issued before it, and behind them the
-    // wave issued the copies of steps j + 2 .. S-1, the epilogue's C-offset
-    // loads (SM + SN * NACC, unconditional; its stores may be fewer,
-    // whole-padding stores are skipped) and, after the epilogue, the copies
-    // of steps S .. S+j-1: at least (S - 2) * PER_WAVE + SM + SN * NACC
-    // operations.  Waiting until at most that many are outstanding
-    // guarantees the copies without waiting for the C stores to drain (a
-    // count below the exact one only waits more; vmcnt stops at 63).  A
-    // tile shorter than S - 1 steps ends the window early: its own
-    // epilogue starts a new one.
-    constexpr int kAfter = (S - 2) * SH::PER_WAVE + TS::SM + TS::SN * MM::NACC;
-    constexpr int kFresh = kAfter < 63 ? kAfter : 63;
+    // fragments this step reads), else this step's; vmcnt completes in
+    // issue order.  Every persistent epilogue waits for all of its wave's
+    // vector-memory operations issued before its first C store (its C-offset
+    // loads and every copy in flight: store_tile's DRAIN), and does not wait
+    // for the stores.  In step j = 0 .. S-2 after an epilogue the awaited
+    // copies were issued before that epilogue (copies run S steps ahead with
+    // double-buffering, S - 1 without, and the step before the epilogue
+    // already waited for the next one's), so they have landed: the step
+    // needs only the barrier, and the C stores drain behind the MFMAs of
+    // these steps.  A tile shorter than S - 1 steps ends the window early:
+    // its own epilogue starts a new one.
     if (PERSIST && fresh > 0) {
-      step_sync<kFresh>();
+      step_sync<63>();
       --fresh;
     } else {
       step_sync<(S - 2) * SH::PER_WAVE>();
@@ -403,7 +392,7 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       store_tile_staged<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale, smem_raw,
                                                SH::LDS_BYTES);
     } else {
-      store_tile<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale);
+      store_tile<P, BM, BN, BK, WM, WN, PERSIST>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale);
     }
     if (!PERSIST) break;
     v += gridDim.x;

@@ -577,8 +577,25 @@ def plan_kind(plan: GettPlan) -> int:
     return kind
 
 
+def tile_tables_digest() -> str:
+    """Digest of every tile table: tuning-database entries name tiles by id,
+    so entries written against other tables are not looked up."""
+    import hashlib
+    text = repr([sorted(t.items()) for t in (TILES, PACKED_TILES, VECTOR_TILES)])
+    return hashlib.sha256(text.encode()).hexdigest()[:12]
+
+
+def tile_shape_of(prec: int, tile: int, kind: int) -> Optional[Tuple[int, int, int]]:
+    """(BM, BN, BK) that tile id `tile` names for a plan of `kind`, None if
+    the id is out of range."""
+    table = (VECTOR_TILES if kind & KIND_VECTOR else PACKED_TILES if kind & KIND_PACKED else TILES).get(prec, [])
+    return tuple(table[tile]) if 0 <= tile < len(table) else None
+
+
 def plan_from_kind(pb: GettProblem, prec: int, tile: int, m_blocked: bool, kind: int, groupM: int = 8) -> GettPlan:
     """The plan a tuning-database entry (tile, m_blocked, kind) describes."""
+    if tile_shape_of(prec, tile, kind) is None:
+        raise ValueError("tile id %d is not in the tile table of this plan kind" % tile)
     if kind & KIND_VECTOR:
         return build_vector_plan(pb, prec, tile, groupM=groupM)
     if kind & KIND_PACKED:

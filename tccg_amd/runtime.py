@@ -274,15 +274,18 @@ class GeneratedGett:
     """A per-contraction library emitted by tccg_amd.codegen (exports gett,
     gett_stream, gett_info, gett_contraction, gett_last_error)."""
 
-    INFO_FIELDS = ("M", "N", "K", "prec", "tile", "BM", "BN", "BK", "modeA", "modeB")
+    INFO_FIELDS = ("M", "N", "K", "prec", "tile", "BM", "BN", "BK", "modeA", "modeB", "sizeA", "sizeB", "sizeC")
 
     def __init__(self, path):
         self.path = Path(path)
         self.lib = ctypes.CDLL(str(self.path))
-        buf = (ctypes.c_int64 * 10)()
+        nf = len(self.INFO_FIELDS)
+        buf = (ctypes.c_int64 * nf)()
         self.lib.gett_info.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        n = self.lib.gett_info(buf, 10)
+        n = self.lib.gett_info(buf, nf)
         self.info = dict(zip(self.INFO_FIELDS, list(buf)[:n]))
+        if n < nf:
+            raise GettRuntimeError("%s reports %d plan fields, %d expected (regenerate it)" % (self.path, n, nf))
         T = ctypes.c_double if self.info["prec"] == F64 else ctypes.c_float
         self.lib.gett_stream.argtypes = [ctypes.c_void_p] * 3 + [T, T, ctypes.c_void_p]
         self.lib.gett_stream.restype = ctypes.c_int
@@ -296,9 +299,13 @@ class GeneratedGett:
 
     def __call__(self, A, B, C, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         torch = _check_cuda()
+        # the library writes C and reads A and B with the extents fixed at
+        # generation time: a tensor of another type, size or device would be
+        # accessed out of bounds
+        dev = cuda_device(torch, None)
+        want = torch.float64 if self.info["prec"] == F64 else torch.float32
         for name, t in (("A", A), ("B", B), ("C", C)):
-            if t.device.type != "cuda" or not t.is_contiguous():
-                raise GettRuntimeError("%s must be a contiguous GPU tensor" % name)
+            check_tensor(name, t, want, self.info["size" + name], dev)
         s = stream if stream is not None else torch.cuda.current_stream()
         rc = self.lib.gett_stream(A.data_ptr(), B.data_ptr(), C.data_ptr(), alpha, beta, ctypes.c_void_p(s.cuda_stream))
         if rc != 0:

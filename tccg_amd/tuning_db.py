@@ -5,9 +5,12 @@ model, the thread count, the float types and beta != 0
 (tccg/sql_util.py:206-239) and stores every timed GETT candidate with its
 measured and estimated GFLOPS (tccg/sql_util.py:97-154); on a later run it
 re-emits the fastest candidate without timing (tccg/tccg.py:201-221).
-Here the key is the (fused) contraction, the device architecture, the
+Here the key is the (fused) contraction, the device architecture together
+with a digest of the tile tables (tccg_amd.plan.tile_tables_digest), the
 precision and beta != 0, and a GETT row records the gfx950 plan parameters
-(tile id, m blocking) with its measured and estimated GFLOP/s.
+(tile id and its shape BM x BN x BK, m blocking, plan kind) with its measured
+and estimated GFLOP/s.  A row is only reused when its tile id still names a
+tile of the recorded shape.
 """
 from __future__ import annotations
 
@@ -30,7 +33,8 @@ CREATE TABLE IF NOT EXISTS gett (
   measurement_id INTEGER NOT NULL REFERENCES measurements(measurement_id),
   name TEXT NOT NULL, tile INTEGER NOT NULL, m_blocked INTEGER NOT NULL,
   gflops REAL NOT NULL, estimated_gflops REAL NOT NULL,
-  kind INTEGER NOT NULL DEFAULT 0
+  kind INTEGER NOT NULL DEFAULT 0,
+  bm INTEGER NOT NULL DEFAULT 0, bn INTEGER NOT NULL DEFAULT 0, bk INTEGER NOT NULL DEFAULT 0
 );
 """
 
@@ -49,9 +53,12 @@ class TuningDB:
         self.con = sqlite3.connect(str(self.path))
         self.con.executescript(SCHEMA)
         cols = [r[1] for r in self.con.execute("PRAGMA table_info(gett)").fetchall()]
-        if "kind" not in cols:   # databases written before plan variants were recorded
-            self.con.execute("ALTER TABLE gett ADD COLUMN kind INTEGER NOT NULL DEFAULT 0")
-            self.con.commit()
+        # databases written before plan variants / tile shapes were recorded
+        # (rows without a shape read as 0 x 0 x 0 and are never reused)
+        for col in ("kind", "bm", "bn", "bk"):
+            if col not in cols:
+                self.con.execute("ALTER TABLE gett ADD COLUMN %s INTEGER NOT NULL DEFAULT 0" % col)
+        self.con.commit()
 
     def close(self) -> None:
         self.con.close()
@@ -75,20 +82,23 @@ class TuningDB:
         return int(cur.lastrowid)
 
     def insert_gett(self, mid: int, name: str, tile: int, m_blocked: bool, gflops: float, estimated: float,
-                    kind: int = 0) -> None:
-        self.con.execute("INSERT INTO gett(measurement_id, name, tile, m_blocked, gflops, estimated_gflops, kind) "
-                         "VALUES (?,?,?,?,?,?,?)", (mid, name, int(tile), int(bool(m_blocked)), float(gflops),
-                                                    float(estimated), int(kind)))
+                    kind: int = 0, shape: Tuple[int, int, int] = (0, 0, 0)) -> None:
+        self.con.execute("INSERT INTO gett(measurement_id, name, tile, m_blocked, gflops, estimated_gflops, kind, "
+                         "bm, bn, bk) VALUES (?,?,?,?,?,?,?,?,?,?)",
+                         (mid, name, int(tile), int(bool(m_blocked)), float(gflops), float(estimated), int(kind),
+                          int(shape[0]), int(shape[1]), int(shape[2])))
         self.con.commit()
 
-    def fastest(self, contraction, device: str, prec: str, beta: float) -> Optional[Tuple[str, int, bool, int, float]]:
-        """(name, tile, m_blocked, kind, gflops) of the fastest stored candidate
-        (kind: tccg_amd.plan.KIND_* bits)."""
+    def fastest(self, contraction, device: str, prec: str,
+                beta: float) -> Optional[Tuple[str, int, bool, int, float, Tuple[int, int, int]]]:
+        """(name, tile, m_blocked, kind, gflops, (BM, BN, BK)) of the fastest
+        stored candidate (kind: tccg_amd.plan.KIND_* bits)."""
         mid = self.measurement_id(contraction, device, prec, beta)
         if mid < 0:
             return None
-        row = self.con.execute("SELECT name, tile, m_blocked, kind, gflops FROM gett WHERE measurement_id=? "
-                               "ORDER BY gflops DESC LIMIT 1", (mid,)).fetchone()
+        row = self.con.execute("SELECT name, tile, m_blocked, kind, gflops, bm, bn, bk FROM gett "
+                               "WHERE measurement_id=? ORDER BY gflops DESC LIMIT 1", (mid,)).fetchone()
         if not row:
             return None
-        return str(row[0]), int(row[1]), bool(row[2]), int(row[3]), float(row[4])
+        return (str(row[0]), int(row[1]), bool(row[2]), int(row[3]), float(row[4]),
+                (int(row[5]), int(row[6]), int(row[7])))

@@ -101,3 +101,29 @@ def test_tuned_contract_gradients():
     L2.backward()
     for g1, g2 in ((A1.grad, A2.grad), (B1.grad, B2.grad)):
         assert ((g1.double() - g2).norm() / g2.norm()).item() < 1e-5
+
+
+def test_gradients_of_permuted_views():
+    """Inputs that are permuted (non-contiguous) views: the forward contracts
+    their contiguous copies and backward returns gradients of the views' shapes."""
+    import torch
+    c_l, a_l, b_l = PATTERNS[0]
+    sizes = {l: 5 for l in set(c_l + a_l + b_l)}
+    c = Contraction.from_labels(c_l, a_l, b_l, sizes, alpha=0.7)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    sa, sb = c.A0.torch_shape(), c.B0.torch_shape()
+    A0 = torch.rand(tuple(reversed(sa)), dtype=torch.float64, device="cuda", generator=g)
+    B0 = torch.rand(tuple(reversed(sb)), dtype=torch.float64, device="cuda", generator=g)
+    A = A0.permute(*reversed(range(A0.dim()))).requires_grad_(False)
+    B = B0.permute(*reversed(range(B0.dim())))
+    assert not A.is_contiguous() and A.shape == sa
+    A.requires_grad_(True)
+    B.requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda x, y: contract(x, y, c, "f64"), (A, B), eps=1e-6, atol=1e-8, rtol=1e-6)
+    L = contract(A, B, c, "f64").sum()
+    L.backward()
+    A2 = A.detach().clone().requires_grad_()
+    B2 = B.detach().clone().requires_grad_()
+    torch.einsum(einsum_eq(c), A2, B2).mul(0.7).sum().backward()
+    assert torch.allclose(A.grad, A2.grad, rtol=1e-12, atol=1e-12)
+    assert torch.allclose(B.grad, B2.grad, rtol=1e-12, atol=1e-12)

@@ -105,3 +105,54 @@ def test_batch_with_packed_problems():
         torch.cuda.synchronize()
         for C, E in zip(Cs, eager):
             assert torch.equal(C, E), "replay %d differs from the eager launch" % r
+
+
+def _checksum(c, A, B):
+    """sum over C of A.B = sum_k (sum over A's free indices of A)(sum over B's
+    free indices of B), in FP64: a size-independent property of a contraction."""
+    ka = [i.label for i in c.A0.indices]
+    kb = [i.label for i in c.B0.indices]
+    k = [l for l in ka if l in kb]
+    ra = np.asarray(A, np.float64).sum(axis=tuple(A.ndim - 1 - ka.index(l) for l in ka if l not in k))
+    rb = np.asarray(B, np.float64).sum(axis=tuple(B.ndim - 1 - kb.index(l) for l in kb if l not in k))
+    # remaining axes are the k indices in reversed A / B order
+    ka_rev = [l for l in reversed(ka) if l in k]
+    kb_rev = [l for l in reversed(kb) if l in k]
+    rb = np.transpose(rb, [kb_rev.index(l) for l in ka_rev])
+    return float((ra * rb).sum()), float((np.abs(ra) * np.abs(rb)).sum())
+
+
+def test_full_batch_every_problem():
+    """The benchmark's batch: 512 contractions of the 4 patterns, every extent
+    uniform over multiples of 8 in [8, 96] (seed 0), packed problems as in the
+    benchmark.  Every problem: against the FP64 einsum oracle on the CPU
+    (problems up to 2 GFLOP) or FP64 torch.einsum on the GPU (the rest, an
+    independent library path), and its element sum against the FP64 checksum."""
+    import torch
+    cs = random_batch(512, seed=0)
+    As, Bs, Cs, hA, hB = make_tensors(cs, 0)
+    b = GettBatch(cs, packed_min_tiles=512).bind(As, Bs, Cs)
+    print("full batch: %d problems, %d on the packed path, %d tiles" % (len(cs), len(b.large), b._bound["nwork"]),
+          flush=True)
+    g = b.capture()
+    g.replay()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, c in enumerate(cs):
+        C = Cs[i]
+        if c.flops() <= 2e9:
+            ref = contract_ref(lab(c.C0), lab(c.A0), lab(c.B0), hA[i], hB[i])
+            err = rel_frobenius(C.cpu().numpy(), ref)
+        else:
+            import string
+            letters = dict(zip(sorted({l for T in (c.C0, c.A0, c.B0) for l in lab(T)}), string.ascii_letters))
+            rev = lambda T: "".join(letters[l] for l in reversed(lab(T)))
+            ref = torch.einsum("%s,%s->%s" % (rev(c.A0), rev(c.B0), rev(c.C0)), As[i].double(), Bs[i].double())
+            err = float((C.double() - ref).norm() / ref.norm())
+            del ref
+        worst = max(worst, err)
+        assert err < 1e-5, (i, i in b.large, str(c), err)
+        s, scale = _checksum(c, hA[i], hB[i])
+        got = float(C.double().sum())
+        assert abs(got - s) <= 1e-5 * scale, (i, str(c), got, s)
+    print("full batch: worst rel. Frobenius error %.3e" % worst, flush=True)

@@ -155,9 +155,31 @@ CREATE TABLE gett (gett_id INTEGER PRIMARY KEY AUTOINCREMENT, measurement_id INT
     c = Contraction.from_tccg(WORKLOADS["example"])
     mid = db.measurement_id(c, "gfx950", "bf16x3", 0.0, create=True)
     db.insert_gett(mid, "a", 1, False, 10.0, 5.0, kind=0)
-    db.insert_gett(mid, "b", 2, True, 30.0, 5.0, kind=3)
-    assert db.fastest(c, "gfx950", "bf16x3", 0.0) == ("b", 2, True, 3, 30.0)
+    db.insert_gett(mid, "b", 2, True, 30.0, 5.0, kind=3, shape=(128, 128, 16))
+    assert db.fastest(c, "gfx950", "bf16x3", 0.0) == ("b", 2, True, 3, 30.0, (128, 128, 16))
     db.close()
+
+
+def test_tuning_db_entries_check_the_tile_shape():
+    """A database row names its tile by id and records the shape: a row whose
+    id names another shape today (a reordered or shortened table) is not
+    reused, and the device key carries a digest of the tile tables."""
+    from tccg_amd import plan as P
+    assert P.tile_shape_of(P.BF16X3, 2, P.KIND_PACKED) == tuple(P.PACKED_TILES[P.BF16X3][2])
+    assert P.tile_shape_of(P.F32, 0, P.KIND_VECTOR) == tuple(P.VECTOR_TILES[P.F32][0])
+    assert P.tile_shape_of(P.F32, 999, 0) is None
+    d = P.tile_tables_digest()
+    saved = list(P.PACKED_TILES[P.F32])
+    try:
+        P.PACKED_TILES[P.F32].reverse()
+        assert P.tile_tables_digest() != d
+    finally:
+        P.PACKED_TILES[P.F32][:] = saved
+    assert P.tile_tables_digest() == d
+    from tccg_amd import Contraction
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    with pytest.raises(ValueError):
+        P.plan_from_kind(c.problem, P.F32, 999, False, P.KIND_PACKED)
 
 
 def test_emit_packed_source():

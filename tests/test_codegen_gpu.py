@@ -80,3 +80,25 @@ def test_generated_packed_library(wl, prec, tol):
         g(tA, tB, tC, 1.0, 0.0)
         torch.cuda.synchronize()
         assert rel_frobenius(tC.cpu().numpy(), ref) < tol
+
+
+def test_generated_library_rejects_bad_tensors():
+    """The generated entry point has its extents baked in: the wrapper checks
+    dtype, element count and device before the launch."""
+    import torch
+    from tccg_amd.runtime import GettRuntimeError
+    lib_path = codegen.GENERATED_DIR / ("lib%s.so" % codegen.workload_library_name("example", F32))
+    g = GeneratedGett(lib_path)
+    c = Contraction.from_tccg(WORKLOADS["example"])
+    assert (g.info["sizeA"], g.info["sizeB"], g.info["sizeC"]) == (c.A0.size(), c.B0.size(), c.C0.size())
+    A, B, C = inputs(c, seed=1)
+    dev = torch.device("cuda", 0)
+    tA, tB, tC = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+    with pytest.raises(TypeError):
+        g(tA.double(), tB, tC)
+    with pytest.raises(GettRuntimeError):
+        g(tA, tB, tC.reshape(-1)[: tC.numel() - 1])
+    with pytest.raises(GettRuntimeError):
+        g(tA, tB, tC.cpu())
+    g(tA, tB, tC)   # the well-formed call still runs
+    torch.cuda.synchronize()
