@@ -13,8 +13,10 @@
  *    and C are DEVICE pointers to dense column-major tensors (leftmost index
  *    stride-one, tccg/tensor.py:8-13); the call is asynchronous on the null
  *    stream.  The generated library additionally exports gett_stream(),
- *    which reports errors and takes a stream, and gett_info().
- *    These are declared by the generated header, see tccg_gett_generated.h.
+ *    which reports errors and takes a stream, gett_info(), gett_contraction()
+ *    and gett_last_error().  These are declared by the header <name>.hpp that
+ *    the generator writes next to the library (tccg_amd/codegen.py,
+ *    emit_header()).
  *
  * 2. The prebuilt, plan-driven library (libtccg_gfx950.so, declared here).
  *    It runs any contraction whose offsets the host planner has tabulated
@@ -97,7 +99,14 @@ typedef struct tccg_gett_plan {
                        /* every aligned group of four m positions is four   */
                        /* consecutive, 4-aligned offsets in both A and C    */
                        /* (or invalid as a whole), any other mode reads and */
-                       /* writes one element at a time                      */
+                       /* writes one element at a time;                     */
+                       /* 4: packed plans only, the store-wave kernel       */
+                       /* (tccg_packed_sw_available()): a persistent grid   */
+                       /* of `grid` workgroups of 512 threads, four waves   */
+                       /* computing tiles through the LDS copy ring and     */
+                       /* four waves writing the finished tiles to C from   */
+                       /* LDS, so that C's write-back overlaps the next     */
+                       /* tiles' compute; grid > 0                          */
   int32_t crun_m;      /* packed plans, one tile per workgroup: when > 0,   */
   int32_t crun_n;      /* C's stride-one index spans crun_m consecutive m   */
                        /* positions and C's next index crun_n consecutive n */
@@ -106,6 +115,13 @@ typedef struct tccg_gett_plan {
                        /* whole blocks and C is 16-byte aligned: the tile   */
                        /* is written as crun_m * crun_n element runs.       */
                        /* 0 for the plain epilogue.                         */
+  int32_t cvec;        /* packed plans with walk 4: 1 when every aligned    */
+                       /* group of four m positions is four consecutive     */
+                       /* elements of C at a 16-byte boundary (offCm[m + j] */
+                       /* = offCm[m] + j, offCm[m] % 4 == 0, or all four    */
+                       /* TCCG_INVALID) and every offCn entry is a multiple */
+                       /* of 4: C is then written with 16-byte stores (C    */
+                       /* must be 16-byte aligned); 0 otherwise             */
 } tccg_gett_plan;
 
 /* Tile shape (BM x BN x BK) of tile configuration `tile` for precision `prec`. */
@@ -153,6 +169,9 @@ int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float 
  * plan's amax as tccg_gett_fp8 does.
  */
 size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan);
+/* 1 when the store-wave kernel (walk 4) is built for packed tile `tile` of
+ * `prec` (FP32-result precisions), else 0. */
+int tccg_packed_sw_available(int prec, int tile);
 int tccg_packed_tile_shape(int prec, int tile, int* BM, int* BN, int* BK);
 int tccg_num_packed_tiles(int prec);
 int tccg_gett_packed_f32(const float* A, const float* B, float* C, float alpha, float beta,

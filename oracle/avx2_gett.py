@@ -2,11 +2,14 @@
 and CPU baseline only.
 
 Classification (m/n/k sets, A/B swap so that the internal A holds C's
-stride-one index) follows tccg/tccg_util.py:212-296, tccg/gett.py:64-89;
-block sizes follow the reference's policy for variant 0 (n-outer):
-mr=24/nr=4 (float), 12/4 (double), mc from getPossibleValues(mr, M, 192|96, 3),
-nc = N when N < 4600, kc from getPossibleValues(1, K, 256, 4)
-(tccg/gett.py:1008-1059, tccg/gett.py:689-698).
+stride-one index) follows tccg/tccg_util.py:212-296, tccg/gett.py:64-89.
+The loop variant and block sizes (mr=24/nr=4 float, 12/4 double; mc, nc, kc)
+come from a candidate of the restated AVX2 planner (oracle/avx2_model.py);
+by default its best-ranked one, the reference's policy
+(tccg/gett.py:961-1150).  The restated kernel blocks the m and n sets as
+ranges of their linearisation in C's order (the reference blocks the split
+sub-index sets that the candidate's name records); a block holds mc (nc)
+positions either way.
 """
 from __future__ import annotations
 
@@ -39,22 +42,10 @@ def _load():
         for name, T in (("avx2_gett_f32", ctypes.c_float), ("avx2_gett_f64", ctypes.c_double)):
             fn = getattr(lib, name)
             fn.argtypes = [P, P, P, T, T, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64] + [P] * 6 + \
-                [ctypes.c_int] * 4
+                [ctypes.c_int] * 5
             fn.restype = ctypes.c_int
         _lib = lib
     return _lib
-
-
-def possible_values(block: int, must: int, target: int, maxn: int) -> List[int]:
-    out = []
-    a = target // block
-    while len(out) < maxn and a >= 1:
-        while must % (a * block) != 0 and a > 1:
-            a -= 1
-        if a * block > 0 and must % (a * block) == 0:
-            out.append(a * block)
-        a -= 1
-    return out
 
 
 def _strides(labels, sizes):
@@ -78,9 +69,11 @@ def _table(order: Sequence[str], sizes: Dict[str, int], stride: Dict[str, int]) 
 
 
 class Avx2Gett:
-    """The restated AVX2 GETT for one contraction (c, a, b label lists)."""
+    """The restated AVX2 GETT for one contraction (c, a, b label lists) with
+    the blocking of `candidate` (an oracle.avx2_model.Avx2Candidate; default:
+    the planner's best-ranked candidate)."""
 
-    def __init__(self, c, a, b, sizes: Dict[str, int], dtype=np.float32):
+    def __init__(self, c, a, b, sizes: Dict[str, int], dtype=np.float32, candidate=None):
         self.c, self.a, self.b, self.sizes, self.dtype = list(c), list(a), list(b), dict(sizes), dtype
         holder_is_a = c[0] in a
         X, Y = (a, b) if holder_is_a else (b, a)
@@ -95,16 +88,21 @@ class Avx2Gett:
         self.tabs = [_table(m, sizes, sx), _table(k, sizes, sx), _table(n, sizes, sy), _table(k, sizes, sy),
                      _table(m, sizes, sc), _table(n, sizes, sc)]
         dbl = dtype == np.float64
-        mr, nr = (12, 4) if dbl else (24, 4)
-        self.mr, self.nr = mr, nr
-        mcs = possible_values(mr, self.M, 96 if dbl else 192, 3)
-        self.mc = mcs[0] if mcs else 0
-        self.nc = self.N if self.N < 4600 else (possible_values(nr, self.N, 4600, 3) or [0])[0]
-        kcs = possible_values(1, self.K, 256, 4)
-        self.kc = kcs[0] if kcs else 0
+        self.mr, self.nr = (12, 4) if dbl else (24, 4)
+        self.variant, self.mc, self.nc, self.kc = 0, 0, 0, 0
+        self.candidate = candidate
+        if candidate is None and self.M % self.mr == 0 and self.N % self.nr == 0:
+            from .avx2_model import candidates_for
+            ranked = candidates_for(c, a, b, sizes, "double" if dbl else "float", 0.0, 1)
+            candidate = ranked[0] if ranked else None
+            self.candidate = candidate
+        if candidate is not None:
+            self.variant, self.mc, self.nc, self.kc = candidate.variant, candidate.mc, candidate.nc, candidate.kc
 
     def supported(self) -> bool:
-        return (self.mc > 0 and self.nc > 0 and self.kc > 0 and self.M % self.mr == 0 and self.N % self.nr == 0)
+        return (self.mc > 0 and self.nc > 0 and self.kc > 0 and self.M % self.mr == 0 and self.N % self.nr == 0
+                and self.M % self.mc == 0 and self.N % self.nc == 0 and self.K % self.kc == 0
+                and self.mc % self.mr == 0 and self.nc % self.nr == 0)
 
     def flops(self) -> float:
         return 2.0 * self.M * self.N * self.K
@@ -120,7 +118,7 @@ class Avx2Gett:
         ptr = lambda x: x.ctypes.data_as(ctypes.c_void_p)
         fn = lib.avx2_gett_f64 if self.dtype == np.float64 else lib.avx2_gett_f32
         rc = fn(ptr(X), ptr(Y), ptr(C), alpha, beta, self.M, self.N, self.K, *[ptr(t) for t in self.tabs],
-                self.mc, self.nc, self.kc, threads)
+                self.variant, self.mc, self.nc, self.kc, threads)
         if rc != 0:
             raise RuntimeError("avx2_gett rejected the blocking (rc=%d)" % rc)
         return C
@@ -137,41 +135,68 @@ def physical_cores() -> int:
     return os.cpu_count() or 1
 
 
-def bench_workload(name: str, seconds: float = 15.0, log=print) -> dict:
-    """Time the restated AVX2 GETT on the named workload: whole contractions,
-    minimum over repetitions, on all physical cores and on one core."""
+def cpu_model() -> str:
     import re
+    try:
+        with open("/proc/cpuinfo") as f:
+            m = re.search(r"model name\s*:\s*(.*)", f.read())
+            return m.group(1).strip() if m else ""
+    except OSError:
+        return ""
+
+
+def _time(fn, budget: float, max_reps: int) -> tuple:
+    best, t_end, reps = float("inf"), time.perf_counter() + budget, 0
+    while reps < 1 or (time.perf_counter() < t_end and reps < max_reps):
+        t0 = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - t0)
+        reps += 1
+    return best, reps
+
+
+def bench_workload(name: str, seconds: float = 5.0, top: int = 3, log=print) -> dict:
+    """The reference's measure-and-select on the host (tccg/tccg.py:987-1058,
+    timing by min over repetitions, tccg/tccg.py:725-737): the restated
+    planner's `top` best-ranked candidates, each timed on all physical cores,
+    the fastest kept and also timed on one core; its result checked against
+    the FP64 einsum oracle (relative Frobenius error <= 1e-5)."""
     from tccg_amd.frontend import parse_tccg  # parser only: labels and sizes
     from tccg_amd.workloads import get
+    from .avx2_model import candidates_for
+    from .einsum_ref import contract_ref, rel_frobenius
     C_, A_, B_, alpha, beta = parse_tccg(get(name))
     c, a, b = ([i.label for i in T.indices] for T in (C_, A_, B_))
     sizes = {i.label: i.size for T in (C_, A_, B_) for i in T.indices}
-    g = Avx2Gett(c, a, b, sizes)
+    ranked = candidates_for(c, a, b, sizes, "float", beta, 16)
     rng = np.random.default_rng(0)
     A = rng.uniform(-1, 1, [sizes[l] for l in reversed(a)]).astype(np.float32)
     B = rng.uniform(-1, 1, [sizes[l] for l in reversed(b)]).astype(np.float32)
     C = np.zeros([sizes[l] for l in reversed(c)], dtype=np.float32)
     cores = physical_cores()
-    out = {}
-    budget = {cores: 0.6 * seconds, 1: 0.4 * seconds}
-    for nt in (cores, 1):
-        best, t_end, reps = float("inf"), time.perf_counter() + budget[nt], 0
-        while reps < 1 or (time.perf_counter() < t_end and reps < 50):
-            t0 = time.perf_counter()
-            g(A, B, C, alpha, beta, threads=nt)
-            best = min(best, time.perf_counter() - t0)
-            reps += 1
-        out[nt] = (g.flops() / best / 1e9, reps, best)
-        log("cpu baseline (restated AVX2 GETT) %s: %d threads %.1f GFLOP/s (min of %d runs, %.3f s)"
-            % (name, nt, out[nt][0], reps, best))
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            m = re.search(r"model name\s*:\s*(.*)", f.read())
-            model = m.group(1).strip() if m else ""
-    except OSError:
-        pass
-    return {"value": round(out[cores][0], 1), "unit": "GFLOP/s", "cores": cores, "kind": "port",
-            "sample": "whole %s contraction (mc=%d nc=%d kc=%d), min over %d runs; 1 core: %.1f GFLOP/s (%d runs); "
-                      "CPU %s" % (name, g.mc, g.nc, g.kc, out[cores][1], out[1][0], out[1][1], model),
-            "one_core_value": round(out[1][0], 1)}
+    timed = []
+    for cand in ranked:
+        g = Avx2Gett(c, a, b, sizes, candidate=cand)
+        if not g.supported():
+            continue
+        t, reps = _time(lambda: g(A, B, C, alpha, beta, threads=cores), seconds * 0.5 / top, 20)
+        timed.append((t, reps, g, cand))
+        log("cpu baseline %s %s: estimated %.1f GFLOP/s (one Haswell core), %d threads %.1f GFLOP/s"
+            % (name, cand.name()[:40], cand.estimate, cores, g.flops() / t / 1e9))
+        if len(timed) >= top:
+            break
+    if not timed:
+        raise ValueError("no candidate of the restated planner runs %s" % name)
+    t_all, reps_all, g, cand = min(timed, key=lambda x: x[0])
+    t_one, reps_one = _time(lambda: g(A, B, C, alpha, beta, threads=1), seconds * 0.5, 10)
+    C[...] = 0
+    g(A, B, C, alpha, beta, threads=cores)
+    err = rel_frobenius(C, contract_ref(c, a, b, A, B, None, alpha, 0.0))
+    if not err <= 1e-5:
+        raise RuntimeError("restated AVX2 GETT misses the einsum oracle on %s: %.3e" % (name, err))
+    gf_all, gf_one = g.flops() / t_all / 1e9, g.flops() / t_one / 1e9
+    log("cpu baseline %s: %s, %d threads %.1f GFLOP/s, 1 thread %.1f GFLOP/s, rel. err %.1e"
+        % (name, cand.name()[:60], cores, gf_all, gf_one, err))
+    return {"workload": name, "candidate": cand.name(), "estimated_gflops_one_haswell_core": round(cand.estimate, 2),
+            "gflops": round(gf_all, 1), "gflops_one_core": round(gf_one, 1), "cores": cores,
+            "candidates_timed": len(timed), "reps": reps_all, "reps_one_core": reps_one, "rel_err_vs_fp64": err}

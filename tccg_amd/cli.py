@@ -225,16 +225,56 @@ def main(argv=None) -> int:
     return 0
 
 
+def harness_inputs(n: int, which: str, dtype, device="cuda"):
+    """The generated harness's deterministic tensor contents
+    (tccg/tccg.py:957-971): A[i] = (10 + (i+1)*17 % 100) / 110,
+    B[i] = (10 + (i+2)*3 % 100) / 110, C[i] = (10 + (i+3)*13 % 100) / 110,
+    the integer part computed exactly, rounded once to FP32 ((float) cast)
+    and divided in double, then stored in the tensor's type."""
+    import torch
+    mul, add = {"A": (17, 1), "B": (3, 2), "C": (13, 3)}[which]
+    i = torch.arange(n, dtype=torch.int64, device=device)
+    v = (10 + ((i + add) * mul) % 100).to(torch.float32).to(torch.float64) / 110.0
+    return v.to(dtype)
+
+
+def harness_equal(x, ref, is_double: bool) -> bool:
+    """The generated harness's check (tccg/tccg.py:843-881): for every element
+    min(|d| / max(|a|, |b|), |d|) <= 1e-4 (float) or 1e-9 (double), where
+    d = |a| - |b| (the harness compares absolute values)."""
+    import torch
+    a = x.double().abs().reshape(-1)
+    b = ref.double().abs().reshape(-1)
+    d = (a - b).abs()
+    rel = torch.where(d > 0, torch.minimum(d / torch.maximum(a, b), d), torch.zeros_like(d))
+    return bool((rel <= (1e-9 if is_double else 1e-4)).all())
+
+
 def _test_inputs(contraction, prec, with_ref: bool):
+    """A, B, C in the generated harness's deterministic contents, and the FP64
+    result alpha * A.B + beta * C when with_ref."""
     import torch
     dt = torch.float64 if prec == F64 else torch.float32
     sa, sb, sc = contraction.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    n = lambda s: int(__import__("numpy").prod(s))
+    A = harness_inputs(n(sa), "A", dt).reshape(sa)
+    B = harness_inputs(n(sb), "B", dt).reshape(sb)
+    C0 = harness_inputs(n(sc), "C", dt).reshape(sc)
     ref = _fp64_reference(contraction, A, B, C0) if with_ref else None
     return A, B, C0, ref
+
+
+def _check_result(name: str, prec: int, C, ref, tol: float) -> bool:
+    """--testing: the reference harness's element check for FP32 / FP64
+    results and the relative Frobenius bar of this backend (both must pass;
+    the reduced-precision forms only the latter)."""
+    err = float((C.double() - ref).norm() / ref.norm())
+    ok = err <= tol
+    if prec in (F32, F64):
+        ok = ok and harness_equal(C, ref, prec == F64)
+    if not ok:
+        print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+    return ok
 
 
 def _time_log(args, contraction, prec):
@@ -262,9 +302,7 @@ def _time_log(args, contraction, prec):
             C.copy_(C0)
             d.launch()
             torch.cuda.synchronize()
-            err = float((C.double() - ref).norm() / ref.norm())
-            if not err <= 1e-5:
-                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+            if not _check_result(name, prec, C, ref, 1e-5):
                 print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
                 return None
         C.copy_(C0)
@@ -300,9 +338,7 @@ def _time_ttgt(args, contraction, prec):
             C.copy_(C0)
             dt_.run(A, B, C)
             torch.cuda.synchronize()
-            err = float((C.double() - ref).norm() / ref.norm())
-            if not err <= tol:
-                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+            if not _check_result(name, prec, C, ref, tol):
                 print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
                 return None
         C.copy_(C0)
@@ -321,12 +357,7 @@ def _time_candidates(args, contraction, prec, plans, db, device):
     import torch
     from .autotune import time_plan
     from .runtime import DevicePlan
-    dt = torch.float64 if prec == F64 else torch.float32
-    sa, sb, sc = contraction.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    C0 = (torch.rand(sc, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
+    A, B, C0, _ = _test_inputs(contraction, prec, False)
     C = C0.clone()
     ref = None
     ref_gflops = -1.0
@@ -347,9 +378,7 @@ def _time_candidates(args, contraction, prec, plans, db, device):
             C.copy_(C0)
             dp.run(A, B, C, contraction.alpha, contraction.beta)
             torch.cuda.synchronize()
-            err = float((C.double() - ref).norm() / ref.norm())
-            if not err <= tol:
-                print(FAIL + "ERROR: version '%s' failed (relative error %.3e)" % (name, err) + ENDC)
+            if not _check_result(name, prec, C, ref, tol):
                 failed += 1
                 continue
         C.copy_(C0)

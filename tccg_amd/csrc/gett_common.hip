@@ -90,6 +90,17 @@ int tccg_vector_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
   return 0;
 }
 
+int tccg_packed_sw_available(int prec, int tile) {
+  using namespace tccg_lib;
+  switch (prec) {
+    case tccg::F32: return sw_available<tccg::F32>(tile) ? 1 : 0;
+    case tccg::BF16X3: return sw_available<tccg::BF16X3>(tile) ? 1 : 0;
+    case tccg::BF16: return sw_available<tccg::BF16>(tile) ? 1 : 0;
+    case tccg::FP8: return sw_available<tccg::FP8>(tile) ? 1 : 0;
+    default: return 0;
+  }
+}
+
 size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan) {
   return tccg_lib::packed_workspace_bytes(prec, plan, nullptr);
 }

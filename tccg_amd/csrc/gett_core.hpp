@@ -84,6 +84,10 @@ struct GettParams {
   // positions, every tile holds whole crunM x crunN blocks, and each block
   // is crunM * crunN contiguous elements of C (staged through LDS).
   int crunM, crunN;
+  // Store-wave kernel: every aligned group of four m positions is four
+  // consecutive elements of C at a 16-byte boundary (or invalid as a whole),
+  // and C is 16-byte aligned: 16-byte stores.
+  int cvec;
 };
 
 template <int P> struct PrecTraits;

@@ -6,9 +6,11 @@
 
 #include <atomic>
 #include <string>
+#include <type_traits>
 
 #include "gett_core.hpp"
 #include "gett_pack.hpp"
+#include "gett_packed_sw.hpp"
 #include "gett_vector.hpp"
 #include "../../include/tccg_gfx950.h"
 
@@ -118,13 +120,13 @@ inline const TileCfg* tiles_of(int prec, int* n) {
 
 constexpr int kMaxLds = 160 * 1024;
 
-// Workgroups of `kern` (256 threads, `lds` bytes) that fit on the current
-// device at once: a walking grid larger than this would run in rounds.
-inline int resident_limit(const void* kern, int lds) {
+// Workgroups of `kern` (`threads` threads, `lds` bytes) that fit on the
+// current device at once: a walking grid larger than this would run in rounds.
+inline int resident_limit(const void* kern, int lds, int threads = 256) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess) return 0;
   return cus * per_cu;
 }
 
@@ -264,6 +266,7 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   prm.amax = plan->amax;
   prm.crunM = 0;
   prm.crunN = 0;
+  prm.cvec = 0;
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
@@ -332,6 +335,73 @@ int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
   return 0;
+}
+
+// Ring stages of the store-wave kernel: FP8 reads its fragments once per
+// k-step (no double-buffering), so two stages suffice and leave LDS for the
+// staged tile.
+template <int P>
+constexpr int sw_stages() { return P == tccg::FP8 ? 2 : kPackedStages; }
+
+// LDS bytes of the store-wave kernel (tccg::SwShape, computed without
+// instantiating it).
+constexpr int sw_lds_bytes(int planes, int esz, int BM, int BN, int BK, int S) {
+  return S * planes * (BM + BN) * BK * esz + BN * (BM + 4) * 4 + (((BM + BN) * 4 + 1023) / 1024) * 1024 * (S + 2);
+}
+
+// Whether the store-wave kernel is built for a packed tile: FP32 results,
+// rows of whole 64-element groups, ring + staged tile + offsets within LDS.
+template <int P, int BM, int BN, int BK>
+constexpr bool sw_built() {
+  using TR = tccg::PrecTraits<P>;
+  return std::is_same<typename TR::TG, float>::value && BM % 64 == 0 && BN % 4 == 0 &&
+         sw_lds_bytes(TR::PLANES, (int)sizeof(typename TR::TL), BM, BN, BK, sw_stages<P>()) <= kMaxLds;
+}
+
+template <int P, int I = 0>
+bool sw_available(int tile) {
+  if constexpr (I >= packed_count(P)) {
+    return false;
+  } else {
+    constexpr TileCfg c = packed_tile(P, I);
+    if (tile == I) return sw_built<P, c.BM, c.BN, c.BK>();
+    return sw_available<P, I + 1>(tile);
+  }
+}
+
+template <int P, int BM, int BN, int BK>
+int launch_packed_sw_kernel(const tccg::GettParams<float>& prm, int grid, hipStream_t stream) {
+  constexpr int ST = sw_stages<P>();
+  using SW = tccg::SwShape<P, BM, BN, BK, ST>;
+  auto kern = tccg::gett_packed_sw_kernel<P, BM, BN, BK, ST>;
+  static std::atomic<uint64_t> lds_set{0};
+  if (int rc = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), SW::LDS_BYTES, lds_set)) return rc;
+  const int cap = resident_limit(reinterpret_cast<const void*>(kern), SW::LDS_BYTES, 512);
+  if (cap > 0 && grid > cap) grid = cap;
+  const int nblocks = prm.tilesM * prm.tilesN;
+  if (grid > nblocks) grid = nblocks;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(512), SW::LDS_BYTES, stream, prm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Store-wave kernel for packed tile id `tile`.
+template <int P, int I = 0>
+int dispatch_packed_sw_tile(int tile, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
+                            hipStream_t s) {
+  if constexpr (I >= packed_count(P)) {
+    return fail(1, "invalid packed tile id");
+  } else {
+    constexpr TileCfg c = packed_tile(P, I);
+    if (tile == I) {
+      if constexpr (sw_built<P, c.BM, c.BN, c.BK>()) {
+        return launch_packed_sw_kernel<P, c.BM, c.BN, c.BK>(prm, grid, s);
+      }
+      return fail(1, "the store-wave kernel (walk 4) is not built for this precision and tile");
+    }
+    return dispatch_packed_sw_tile<P, I + 1>(tile, prm, grid, s);
+  }
 }
 
 // GETT kernel over packed operands for packed tile id `tile` (I walks the
@@ -455,11 +525,23 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   prm.amax = plan->amax;
   prm.crunM = plan->crun_m;
   prm.crunN = plan->crun_n;
+  prm.cvec = plan->cvec;
   if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
     return fail(1, "crun_m and crun_n must both be positive or both zero");
   if (plan->crun_m > 0 && !aligned16(C)) return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
+  if (plan->walk == 4) {
+    if (plan->grid <= 0) return fail(1, "the store-wave kernel (walk 4) needs grid > 0");
+    if (plan->crun_m > 0) return fail(1, "the store-wave kernel (walk 4) has no run epilogue (crun_m must be 0)");
+    if (plan->cvec != 0 && plan->cvec != 1) return fail(1, "cvec must be 0 or 1");
+    if (plan->cvec && !aligned16(C)) return fail(1, "cvec == 1 needs a 16-byte aligned C");
+    if constexpr (std::is_same<TG, float>::value) {
+      return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
+    }
+    return fail(1, "the store-wave kernel (walk 4) writes FP32 results only");
+  }
+  if (plan->walk != 0) return fail(1, "packed plans take walk 0 or 4");
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
 }
 

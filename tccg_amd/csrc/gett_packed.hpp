@@ -73,7 +73,8 @@ struct PackedShape {
   static_assert(BYTES_A % 1024 == 0 && BYTES_B % 1024 == 0, "k-step blocks are whole kilobytes");
   static_assert((NA + NB) % 4 == 0, "copies divide evenly over the four waves");
   static_assert((BK * (int)sizeof(TL)) % 32 == 0, "rows of whole 32-byte sub-rows");
-  static_assert(S >= 3 && S * PER_WAVE < 64, "ring depth");
+  // copies run S steps ahead with fragment double-buffering (S - 1 without, FP8)
+  static_assert(S >= (P == FP8 ? 2 : 3) && S * PER_WAVE < 64, "ring depth");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS capacity");
 };
 

@@ -85,6 +85,24 @@ def packed_ring_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
     return 3 * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec]
 
 
+def sw_lds_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
+    """LDS of the store-wave kernel (sw_lds_bytes in gett_launch.hpp): the
+    copy ring (2 stages for FP8, else 3), the staged FP32 tile (BN rows of
+    BM + 4 floats) and S + 2 kilobyte-rounded buffers of the tile's C offsets."""
+    S = 2 if prec == FP8 else 3
+    return (S * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec] + BN * (BM + 4) * 4
+            + (-(-(BM + BN) * 4 // 1024)) * 1024 * (S + 2))
+
+
+def sw_built(prec: int, tile: int) -> bool:
+    """Whether the store-wave kernel (walk 4) exists for packed tile `tile`
+    (tccg_packed_sw_available() of the library)."""
+    if prec == F64:
+        return False
+    BM, BN, BK = PACKED_TILES[prec][tile]
+    return BM % 64 == 0 and BN % 4 == 0 and sw_lds_bytes(prec, BM, BN, BK) <= 160 * 1024
+
+
 class UnsupportedContraction(ValueError):
     """The contraction cannot be expressed as a GETT (e.g. batch indices)."""
 
@@ -305,8 +323,10 @@ class GettPlan:
     grid: int = 0          # 0: one workgroup per tile; > 0: persistent walk
     packed: bool = False   # operands converted once into tile-major planes
     walk: int = 0          # 2: streaming walk over `grid` workgroups (single k-step);
-                           # 3: vector micro-kernel (tile indexes VECTOR_TILES)
+                           # 3: vector micro-kernel (tile indexes VECTOR_TILES);
+                           # 4: packed, store-wave kernel over `grid` workgroups
     crun: Tuple[int, int] = (0, 0)  # packed: (E0, E1) blocks of C written as contiguous runs
+    cvec: int = 0          # store-wave kernel: C written in 16-byte groups of four m positions
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -340,6 +360,8 @@ class GettPlan:
         walk = ""
         if self.walk == 3:
             walk = " vector"
+        elif self.walk == 4:
+            walk = " storewaves grid=%d%s" % (self.grid, " cvec" if self.cvec else "")
         elif self.walk == 2:
             walk = " stream grid=%d" % self.grid
         elif self.grid:
@@ -386,8 +408,17 @@ def _ext(pb: GettProblem, labels: Sequence[str]) -> List[int]:
     return out
 
 
+def _cvec_ok(offCm: np.ndarray, offCn: np.ndarray) -> bool:
+    """C can be written in 16-byte groups: every aligned group of four m
+    positions is all invalid or four consecutive offsets from a multiple of
+    4, and every valid n offset is a multiple of 4."""
+    cn = offCn.astype(np.int64)
+    cn = cn[cn != np.int64(INVALID)]
+    return _vector_ok_free(offCm, np.arange(offCm.size), offCm.size) and bool(np.all(cn % 4 == 0))
+
+
 def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
-                      persistent: bool = False) -> GettPlan:
+                      persistent: bool = False, store_waves: bool = False) -> GettPlan:
     """Plan of the packed path: the reference's pack-then-macro-kernel GETT
     (tccg/gett.py:346-391, tccg/gett.py:1092-1098) with each operand packed
     once.  Plain linearisations (the pack kernel reads every element once, so
@@ -413,6 +444,15 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
     ident_n = np.arange(BN, dtype=np.int32)
     plan = GettPlan(pb, prec, tile, BM, BN, BK, lin_m, lin_n, lin_k, offAm, offAk, offBn, offBk, offCm, offCn,
                     ident_m, ident_n, modeA, modeB, groupM, packed=True)
+    if store_waves:
+        # store-wave kernel: persistent, one or two workgroups per CU by LDS
+        if not sw_built(prec, tile):
+            raise ValueError("the store-wave kernel is not built for %s packed tile %d" % (PREC_NAMES[prec], tile))
+        per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // sw_lds_bytes(prec, BM, BN, BK)))
+        plan.walk = 4
+        plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
+        plan.cvec = int(_cvec_ok(offCm, offCn))
+        return plan
     if not persistent:
         esz = 8 if prec == F64 else 4
         ring = packed_ring_bytes(prec, BM, BN, BK)
@@ -563,14 +603,16 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
 
 # Plan variants as stored by the tuning database: bit 0 packed operands,
 # bit 1 persistent walk, bit 2 streaming walk, bit 3 vector micro-kernel.
-KIND_PACKED, KIND_PERSISTENT, KIND_STREAM, KIND_VECTOR = 1, 2, 4, 8
+KIND_PACKED, KIND_PERSISTENT, KIND_STREAM, KIND_VECTOR, KIND_STORE_WAVES = 1, 2, 4, 8, 16
 
 
 def plan_kind(plan: GettPlan) -> int:
     if plan.walk == 3:
         return KIND_VECTOR
     kind = KIND_PACKED if plan.packed else 0
-    if plan.walk == 2:
+    if plan.walk == 4:
+        kind |= KIND_STORE_WAVES
+    elif plan.walk == 2:
         kind |= KIND_STREAM
     elif plan.grid:
         kind |= KIND_PERSISTENT
@@ -599,7 +641,8 @@ def plan_from_kind(pb: GettProblem, prec: int, tile: int, m_blocked: bool, kind:
     if kind & KIND_VECTOR:
         return build_vector_plan(pb, prec, tile, groupM=groupM)
     if kind & KIND_PACKED:
-        return build_packed_plan(pb, prec, tile, groupM=groupM, persistent=bool(kind & KIND_PERSISTENT))
+        return build_packed_plan(pb, prec, tile, groupM=groupM, persistent=bool(kind & KIND_PERSISTENT),
+                                 store_waves=bool(kind & KIND_STORE_WAVES))
     return build_plan(pb, prec, tile, m_blocked=m_blocked, groupM=groupM,
                       persistent=bool(kind & KIND_PERSISTENT), stream=bool(kind & KIND_STREAM))
 
@@ -644,8 +687,11 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
         t_valu = plan.padded_flops() / (0.5 * hw["valu"][plan.prec])
         t = max(t_hbm, t_valu)
         return t, (0.0 if plan.modeA == FREE4 else t)
-    lds = (packed_ring_bytes(plan.prec, plan.BM, plan.BN, plan.BK) if plan.packed
-           else lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK))
+    if plan.walk == 4:
+        lds = sw_lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK)
+    else:
+        lds = (packed_ring_bytes(plan.prec, plan.BM, plan.BN, plan.BK) if plan.packed
+               else lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK))
     per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
     slots = hw["cus"] * per_cu
     ntiles = plan.ntiles()
@@ -673,7 +719,13 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     elif plan.grid:
         t_fixed = 0.4e-6 + t_store * 0.6
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
-    t = rounds * (t_mfma + t_fixed) + t_pack
+    if plan.walk == 4:
+        # store waves write C while the compute waves run the next tile: the
+        # slower of the two per tile (16-byte groups store at a higher rate)
+        t_sw = esz * plan.BM * plan.BN / ((0.8 if plan.cvec else 0.5) * hw["hbm_bytes_per_s"] / slots)
+        t = rounds * (max(t_mfma, t_sw) + 0.3e-6) + t_pack
+    else:
+        t = rounds * (t_mfma + t_fixed) + t_pack
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
     t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
     return max(t, t_hbm), t
@@ -707,6 +759,8 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
             pp = build_packed_plan(pb, prec, tile, groupM=groupM, persistent=True)
             if pp.grid:
                 plans.append(pp)
+        if sw_built(prec, tile):
+            plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, store_waves=True))
     # ties at the HBM floor are broken by the time the tile work alone would take
     plans.sort(key=lambda p: _estimate(p))
     return plans

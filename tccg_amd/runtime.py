@@ -33,7 +33,7 @@ class _CPlan(ctypes.Structure):
         ("tile", ctypes.c_int32), ("modeA", ctypes.c_int32), ("modeB", ctypes.c_int32),
         ("groupM", ctypes.c_int32), ("grid", ctypes.c_int32),
         ("amax", ctypes.c_void_p), ("walk", ctypes.c_int32),
-        ("crun_m", ctypes.c_int32), ("crun_n", ctypes.c_int32),
+        ("crun_m", ctypes.c_int32), ("crun_n", ctypes.c_int32), ("cvec", ctypes.c_int32),
     ]
 
 
@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_
                     "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked",
                     "tccg_num_vector_tiles", "tccg_vector_tile_shape",
                     "tccg_num_transpose_tiles", "tccg_transpose_tile_shape", "tccg_transpose_f32",
-                    "tccg_transpose_f64"]
+                    "tccg_transpose_f64", "tccg_packed_sw_available"]
 
 _PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
               BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
@@ -109,6 +109,13 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
         lib.tccg_gett_batch_f32.restype = ctypes.c_int
         lib.tccg_batch_variant.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 5
         lib.tccg_last_error.restype = ctypes.c_char_p
+        lib.tccg_packed_sw_available.argtypes = [ctypes.c_int, ctypes.c_int]
+        from .plan import sw_built
+        for prec, tiles in PACKED_TILES.items():
+            for t in range(len(tiles)):
+                if bool(lib.tccg_packed_sw_available(prec, t)) != sw_built(prec, t):
+                    raise GettRuntimeError("store-wave availability of packed tile %d/%d disagrees with the host"
+                                           % (prec, t))
         # The Python tile tables must agree with the library's.
         for table, count, shape_of, kind in ((TILES, lib.tccg_num_tiles, lib.tccg_tile_shape, "tile"),
                                               (PACKED_TILES, lib.tccg_num_packed_tiles, lib.tccg_packed_tile_shape,
@@ -197,6 +204,7 @@ class DevicePlan:
         c.walk = plan.walk
         # the run epilogue is built for one tile per workgroup
         c.crun_m, c.crun_n = plan.crun if not plan.grid else (0, 0)
+        c.cvec = plan.cvec
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
         self.workspace = None

@@ -1,10 +1,11 @@
-"""Index algebra: prime factors, block-size candidates, index-set splitting (CPU)."""
+"""The restated AVX2 planner's index algebra (oracle/avx2_model.py): prime
+factors, block-size candidates, index-set splitting (CPU)."""
 import itertools
 
 import numpy as np
 import pytest
 
-from tccg_amd.algebra import get_possible_values, get_perm, prime_factors, split_index, split_index_set, transpose_args
+from oracle.avx2_model import get_possible_values, get_perm, prime_factors, split_index, split_index_set, transpose_args
 from tccg_amd.ir import Index, Tensor
 
 

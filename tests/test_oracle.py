@@ -6,7 +6,8 @@ import os
 import numpy as np
 import pytest
 
-from oracle.avx2_gett import Avx2Gett, possible_values
+from oracle.avx2_gett import Avx2Gett
+from oracle.avx2_model import get_possible_values as possible_values
 from oracle.einsum_ref import contract_ref, reference_equal, reference_inputs, rel_frobenius, subscripts
 from oracle.naive import naive_contract
 
@@ -98,12 +99,35 @@ def test_reference_equal_semantics():
     # the reference compares absolute values: a sign flip passes its check
     assert reference_equal(x, -x, False)
     assert reference_equal(x.astype(np.float64), x.astype(np.float64) * (1 + 5e-10), True)
+    # near zero the absolute difference counts (min of relative and absolute)
+    assert reference_equal(np.array([0.0]), np.array([5e-5]), False)
+    assert not reference_equal(np.array([0.0]), np.array([5e-4]), False)
 
 
-def test_reference_inputs_formula():
-    A = reference_inputs(5, "A")
-    np.testing.assert_allclose(A, [(10 + ((i + 1) * 17) % 100) / 110.0 for i in range(5)], rtol=1e-7)
+def test_reference_inputs_known_values():
+    # hand-computed: A[0] = (10 + 17) / 110, A[5] = (10 + 102 % 100) / 110,
+    # B[0] = (10 + 6) / 110, C[29] = (10 + 416 % 100) / 110
+    assert reference_inputs(6, "A")[[0, 5]].tolist() == [np.float32(27 / 110.0), np.float32(12 / 110.0)]
+    assert reference_inputs(1, "B")[0] == np.float32(16 / 110.0)
+    assert reference_inputs(30, "C")[29] == np.float32(26 / 110.0)
     assert 0.09 <= reference_inputs(1000, "C").min() and reference_inputs(1000, "B").max() <= 1.0
+
+
+def test_cli_harness_matches_the_oracle():
+    """The CLI's restatement of the generated harness (inputs and element
+    check, tccg_amd/cli.py) agrees with the oracle's on the CPU."""
+    import torch
+    from tccg_amd.cli import harness_equal, harness_inputs
+    for which in "ABC":
+        for dt, ndt in ((torch.float32, np.float32), (torch.float64, np.float64)):
+            got = harness_inputs(5000, which, dt, device="cpu").numpy()
+            assert np.array_equal(got, reference_inputs(5000, which, ndt)), (which, dt)
+    rng = np.random.default_rng(4)
+    for scale in (1e-6, 5e-5, 2e-4, 1e-3):
+        x = rng.uniform(-1, 1, 4096)
+        y = x * (1 + scale * rng.standard_normal(4096))
+        for dbl in (False, True):
+            assert harness_equal(torch.from_numpy(y), torch.from_numpy(x), dbl) == reference_equal(y, x, dbl)
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
@@ -127,8 +151,31 @@ def test_avx2_gett_restatement(dt, case):
 
 
 def test_avx2_blocking_follows_reference_policy():
-    # config (i): M=N=5184 (mc from {192,144,96}, nc = N < 4600? no -> 4600-ish), K=2304 (kc 256)
+    # config (i): the restated planner's best-ranked candidate is the m-outer
+    # variant with mc from getPossibleValues(24, 5184, 4600, 3), nc from
+    # getPossibleValues(4, 5184, 192, 3), kc from getPossibleValues(1, 2304, 256, 4)
     g = Avx2Gett(["a", "b", "c", "d"], ["a", "e", "b", "f"], ["d", "f", "c", "e"],
                  dict(a=72, b=72, c=72, d=72, e=48, f=48))
-    assert g.mc == 192 and g.kc == 256 and g.nc == possible_values(4, 5184, 4600, 3)[0] == 2592
+    assert (g.variant, g.mc, g.nc, g.kc) == (1, 2592, 192, 256)
+    assert g.mc == possible_values(24, 5184, 4600, 3)[0] and g.nc == possible_values(4, 5184, 192, 3)[0]
+    assert g.kc == possible_values(1, 2304, 256, 4)[0]
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_avx2_gett_both_variants(variant):
+    from oracle.avx2_model import candidates_for
+    c, a, b = ["a", "b", "c"], ["c", "k", "a"], ["k", "b"]
+    sizes = dict(a=48, b=24, c=20, k=96)
+    cands = [x for x in candidates_for(c, a, b, sizes, "float", 0.0, -1) if x.variant == variant]
+    assert cands
+    rng = np.random.default_rng(variant)
+    A = rng.uniform(-1, 1, [sizes[l] for l in reversed(a)]).astype(np.float32)
+    B = rng.uniform(-1, 1, [sizes[l] for l in reversed(b)]).astype(np.float32)
+    C0 = rng.uniform(-1, 1, [sizes[l] for l in reversed(c)]).astype(np.float32)
+    ref = contract_ref(c, a, b, A, B, C0, 0.5, 2.0)
+    for cand in cands[:4]:
+        g = Avx2Gett(c, a, b, sizes, candidate=cand)
+        C = C0.copy()
+        g(A, B, C, 0.5, 2.0, threads=3)
+        assert rel_frobenius(C, ref) < 1e-5, cand.name()
     assert not Avx2Gett(["a", "b"], ["a", "k"], ["k", "b"], dict(a=20, b=8, k=4)).supported()

@@ -68,3 +68,58 @@ def test_every_plan_addresses_the_contraction(seed):
         out = emulate(plan, A, B, C, 1.25, -0.5)
         assert rel_frobenius(out, ref) < 1e-13, (str(c), plan.describe())
     assert any(p for p, _, _ in kinds) and any(not p for p, _, _ in kinds), kinds
+
+
+def emulate_store_waves(plan, C_exact, C_init):
+    """The store-wave kernel's write of a finished tile: with cvec, aligned
+    groups of four m positions are written as four consecutive elements from
+    offCm[m4] + offCn[n] (16-byte aligned); without, element by element."""
+    cm = plan.offCm.astype(np.int64)
+    cn = plan.offCn.astype(np.int64)
+    out = C_init.reshape(-1).copy()
+    exact = C_exact.reshape(-1)
+    for n in range(cn.size):
+        if cn[n] == INV:
+            continue
+        for m4 in range(0, cm.size, 4):
+            if plan.cvec:
+                if cm[m4] == INV:
+                    continue
+                base = cm[m4] + cn[n]
+                assert base % 4 == 0
+                # the value of position (m4 + j, n) lands at base + j
+                for j in range(4):
+                    out[base + j] = exact[cm[m4 + j] + cn[n]]
+            else:
+                for j in range(4):
+                    if cm[m4 + j] != INV:
+                        out[cm[m4 + j] + cn[n]] = exact[cm[m4 + j] + cn[n]]
+    return out.reshape(C_init.shape)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_store_wave_groups(seed):
+    """Every store-wave plan's C writes (16-byte groups when cvec) land on the
+    right elements; contractions whose C has a stride-one extent divisible by
+    4 get cvec plans."""
+    from tccg_amd.plan import BF16X3, PACKED_TILES, build_packed_plan, sw_built
+    rng = np.random.default_rng(100 + seed)
+    c = random_contraction(rng)
+    if seed % 3 == 0:   # make C's stride-one extent a multiple of 4
+        sizes = {i.label: i.size for T in (c.C0, c.A0, c.B0) for i in T.indices}
+        sizes[c.C0.indices[0].label] = 4 * int(rng.integers(1, 4))
+        labels = lambda T: [i.label for i in T.indices]
+        c = Contraction.from_labels(labels(c.C0), labels(c.A0), labels(c.B0), sizes)
+    sc = c.shapes()[2]
+    exact = rng.uniform(-1, 1, sc)
+    init = np.full(sc, np.nan)
+    for prec in (F32, BF16X3):
+        for tile in range(len(PACKED_TILES[prec])):
+            if not sw_built(prec, tile):
+                continue
+            plan = build_packed_plan(c.problem, prec, tile, store_waves=True)
+            assert plan.walk == 4 and plan.grid > 0
+            if seed % 3 == 0:
+                assert plan.cvec, plan.describe()
+            out = emulate_store_waves(plan, exact, init)
+            assert np.array_equal(out, exact), (str(c), plan.describe())

@@ -12,9 +12,11 @@ product set, the part split, the e4m3 rounding or the scales shows:
   256 and k = 96 sums stay below 2^25 (17 significant bits); a quarter of the
   values sit exactly halfway between two e4m3 values (round-to-even ties).
 Every candidate plan of each kernel kind runs (gather, packed, persistent,
-run epilogue).  At the full precision-contraction size the GPU results must
-be within 1e-6 relative Frobenius error of the emulations (FP32 accumulation
-of 96-term sums).
+run epilogue, store waves).  At the full precision-contraction size the GPU
+results must be within 1e-6 (split-BF16: FP32 accumulation of 96-term sums)
+or 1e-4 (FP8) relative Frobenius error of the emulations.  The FP8 MFMA sums
+its 128 products with fewer bits than FP32 (1.5e-5 relative on these N(0,1)
+inputs); a wrong rounding mode or scale moves the result by about 1e-2.
 """
 import numpy as np
 import pytest
@@ -106,4 +108,4 @@ def test_precision_contraction_full_size_matches_emulation(prec):
     emul = (split_bf16_contract if prec == BF16X3 else fp8_contract)(*_labels(c), A, B)
     err = rel_frobenius(got, emul)
     print("rel. Frobenius error vs emulation %.3e" % err, flush=True)
-    assert err <= 1e-6
+    assert err <= (1e-6 if prec == BF16X3 else 1e-4)
