@@ -69,6 +69,19 @@ def fp64_reference(c, A, B):
     return torch.einsum(_einsum_eq(c), A.double().contiguous(), B.double().contiguous()) * c.alpha
 
 
+def seeded_inputs(c, dt, dist: str = "uniform", seed: int = 0):
+    """A then B drawn with numpy.random.default_rng(seed) in FP64 (iid
+    uniform[-1, 1] or N(0, 1)), cast to dt, uploaded to the GPU."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    sa, sb, _ = c.shapes()
+    draw = (lambda s: rng.uniform(-1.0, 1.0, s)) if dist == "uniform" else (lambda s: rng.standard_normal(s))
+    A = torch.from_numpy(draw(sa)).to(dt).cuda()
+    B = torch.from_numpy(draw(sb)).to(dt).cuda()
+    return A, B
+
+
 def timed_steps(dp, A, B, C, steps: int, warmup: int):
     """(wall seconds of `steps` back-to-back calls bracketed by synchronize,
     average per-call GPU time in seconds from HIP events recorded on the
@@ -177,11 +190,8 @@ def run_form(name, wl, prec, args, log):
 
     c = Contraction.from_tccg(get(wl))
     dt = torch.float64 if prec == "f64" else torch.float32
-    sa, sb, sc = c.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = (torch.rand(sa, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    B = (torch.rand(sb, dtype=torch.float64, device="cuda", generator=g) * 2 - 1).to(dt)
-    C = torch.zeros(sc, dtype=dt, device="cuda")
+    A, B = seeded_inputs(c, dt, "normal" if wl == "abcdef-dgfa-gceb" else "uniform")
+    C = torch.zeros(c.shapes()[2], dtype=dt, device="cuda")
     from tccg_amd.plan import PREC_BY_NAME
     p = PREC_BY_NAME[prec]
     t0 = time.perf_counter()
@@ -303,9 +313,7 @@ def run_ttgt(args, log):
     cand = best_candidates(ttgt_candidates(c.problem, 4), 1)[0]
     d = DeviceTtgt(c, cand, F32, tune=True)
     sa, sb, sc = c.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = torch.rand(sa, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
-    B = torch.rand(sb, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    A, B = seeded_inputs(c, torch.float32)
     C = torch.zeros(sc, dtype=torch.float32, device="cuda")
     t = time_plan(d, A, B, C, 1.0, 0.0, reps=max(3, args.steps // 4))
     C.zero_()
@@ -410,9 +418,7 @@ def run_log(args, log):
     c = Contraction.from_tccg(get("abcd-aebf-dfce"))
     cand = best_log_candidates(log_candidates(c.problem), 1)[0]
     sa, sb, sc = c.shapes()
-    g = torch.Generator(device="cuda").manual_seed(0)
-    A = torch.rand(sa, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
-    B = torch.rand(sb, dtype=torch.float64, device="cuda", generator=g).mul_(2).sub_(1).float()
+    A, B = seeded_inputs(c, torch.float32)
     C = torch.zeros(sc, dtype=torch.float32, device="cuda")
     d = DeviceLoG(c, cand, A, B, C, 1.0, 0.0)
     t = time_plan(d, A, B, C, reps=3)
@@ -484,16 +490,32 @@ def profiled_traffic(workload: str, prec: str, plan: str):
     return None
 
 
+CPU_CONFIGS = ("cpu48", "abcd-aebf-dfce", "abc-akb-kc")
+
+
 def cpu_baseline(args, log):
-    """The oracle's AVX2 GETT restatement on the host cores, on a bounded sample
-    of the headline workload (whole contraction repeated for about
-    --cpu-sample-seconds)."""
+    """The oracle's AVX2 GETT restatement on the host cores: the CPU
+    configuration (C[a,b,c,d] = A[a,k,c] * B[k,d,b], 48/64) and both plain FP32
+    GPU configurations, each a bounded sample (the restated planner's best
+    three candidates timed on all physical cores, the fastest also on one core,
+    about --cpu-sample-seconds / 3 per configuration), checked against the
+    FP64 einsum oracle.  `value` is the compute-bound configuration's
+    all-core rate (the bench line's workload)."""
     try:
         from oracle import avx2_gett
     except Exception as e:  # the restatement is optional for the GPU line
         log("cpu baseline unavailable: %s" % e)
         return None
-    return avx2_gett.bench_workload("abcd-aebf-dfce", seconds=args.cpu_sample_seconds, log=log)
+    per = {wl: avx2_gett.bench_workload(wl, seconds=args.cpu_sample_seconds / len(CPU_CONFIGS), log=log)
+           for wl in CPU_CONFIGS}
+    head = per["abcd-aebf-dfce"]
+    return {"value": head["gflops"], "unit": "GFLOP/s", "cores": head["cores"], "kind": "port",
+            "sample": ("whole contractions, min over repetitions after the restated planner's top-3 candidates "
+                       "were timed; configurations: %s" % ", ".join(
+                           "%s %.1f GFLOP/s (%d cores) / %.1f (1 core) [%s]" % (
+                               wl, r["gflops"], r["cores"], r["gflops_one_core"], r["candidate"].split("_mr")[0])
+                           for wl, r in per.items())),
+            "one_core_value": head["gflops_one_core"], "cpu": avx2_gett.cpu_model(), "configs": per}
 
 
 def main():
@@ -575,6 +597,9 @@ def main():
         achieved_tf = head["kernel_gflops"] / 1e3
         roofline = {"bound": "mfma", "achieved": round(achieved_tf, 2), "peak": PEAK_TFLOPS[head["prec"]],
                     "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_TFLOPS[head["prec"]], 4), "traffic": traffic}
+    # the time behind `achieved`: HIP events around graph replays of the GETT kernel alone
+    roofline["time_source"] = ("hip events, graph replay of %d kernel launches: %.4f ms per launch"
+                               % (args.steps, head["kernel_ms"]))
     line = {
         "metric": METRIC,
         "value": round(head["gflops"], 1),
@@ -587,7 +612,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": head["prec"],
-        "data": "synthetic: iid uniform[-1,1] (torch generator seed 0), operands resident in HBM",
+        "data": "synthetic: iid uniform[-1,1] from numpy default_rng(0), A then B, operands resident in HBM",
         "config": {"workload": head["workload"], "M": head["M"], "N": head["N"], "K": head["K"],
                    "alpha": 1.0, "beta": 0.0, "plan": head["plan"]},
         "roofline": roofline,

@@ -114,7 +114,13 @@ typedef struct tccg_gett_plan {
                        /* offCn[n + b] = offCn[n] + b * crun_m), tiles hold */
                        /* whole blocks and C is 16-byte aligned: the tile   */
                        /* is written as crun_m * crun_n element runs.       */
-                       /* 0 for the plain epilogue.                         */
+                       /* 0 for the plain epilogue.  Walk 4: crun_m > 0     */
+                       /* selects the run store order: every aligned run of */
+                       /* crun_m m positions (a multiple of 4 dividing BM)  */
+                       /* is crun_m consecutive elements of C from a        */
+                       /* 16-byte boundary or invalid as a whole; the store */
+                       /* waves write one run's n rows after another (crun_n*/
+                       /* is not read); 0 selects the row order.            */
   int32_t cvec;        /* packed plans with walk 4: 1 when every aligned    */
                        /* group of four m positions is four consecutive     */
                        /* elements of C at a 16-byte boundary (offCm[m + j] */

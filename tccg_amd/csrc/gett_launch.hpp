@@ -49,14 +49,16 @@ constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192
 // pack_kernel, then copied tile by tile); ids index these arrays.  Every BM
 // and BN is a multiple of the 64-row pack block.
 constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
-                                       {192, 192, 16, 2, 2},  {128, 64, 16, 2, 2},  {64, 128, 16, 2, 2}};
+                                       {192, 192, 16, 2, 2},  {128, 64, 16, 2, 2},  {64, 128, 16, 2, 2},
+                                       {192, 64, 16, 2, 2}};
 constexpr TileCfg kF64PackedTiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}};
 constexpr TileCfg kBfPackedTiles[] = {{128, 128, 16, 2, 2}, {128, 128, 32, 2, 2}, {256, 128, 16, 2, 2},
-                                      {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2},   {192, 192, 16, 2, 2}};
+                                      {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2},   {192, 192, 16, 2, 2},
+                                      {192, 64, 16, 2, 2}};
 constexpr TileCfg kBf1PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 32, 2, 2}, {128, 256, 32, 2, 2},
-                                       {64, 64, 32, 2, 2},   {192, 192, 32, 2, 2}};
+                                       {64, 64, 32, 2, 2},   {192, 192, 32, 2, 2},  {192, 64, 32, 2, 2}};
 constexpr TileCfg kF8PackedTiles[] = {{128, 128, 128, 2, 2}, {256, 128, 128, 2, 2}, {64, 64, 128, 2, 2},
-                                      {192, 192, 128, 2, 2}};
+                                      {192, 192, 128, 2, 2}, {192, 64, 128, 2, 2}};
 
 // Tiles of the vector micro-kernel (walk 3, FP32 and FP64): BM m positions
 // (a multiple of 1024: 256 threads x 4 consecutive m x MV groups) by BN n
@@ -337,16 +339,25 @@ int launch_packed_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG
   return 0;
 }
 
-// Ring stages of the store-wave kernel: FP8 reads its fragments once per
-// k-step (no double-buffering), so two stages suffice and leave LDS for the
-// staged tile.
-template <int P>
-constexpr int sw_stages() { return P == tccg::FP8 ? 2 : kPackedStages; }
-
 // LDS bytes of the store-wave kernel (tccg::SwShape, computed without
 // instantiating it).
 constexpr int sw_lds_bytes(int planes, int esz, int BM, int BN, int BK, int S) {
   return S * planes * (BM + BN) * BK * esz + BN * (BM + 4) * 4 + (((BM + BN) * 4 + 1023) / 1024) * 1024 * (S + 2);
+}
+
+// Ring stages of the store-wave kernel: the deepest ring (at most 4 stages)
+// that fits LDS beside the staged tile, so that short k-steps still have
+// copies in flight far enough ahead; at least 3 with fragment
+// double-buffering, 2 without (FP8); 0 when none fits.
+template <int P, int BM, int BN, int BK>
+constexpr int sw_stages() {
+  using TR = tccg::PrecTraits<P>;
+  constexpr int esz = (int)sizeof(typename TR::TL);
+  constexpr int per_wave = TR::PLANES * (BM + BN) * BK * esz / 1024 / 4;
+  constexpr int smin = P == tccg::FP8 ? 2 : 3;
+  for (int S = 4; S >= smin; --S)
+    if (sw_lds_bytes(TR::PLANES, esz, BM, BN, BK, S) <= kMaxLds && S * per_wave < 64) return S;
+  return 0;
 }
 
 // Whether the store-wave kernel is built for a packed tile: FP32 results,
@@ -355,7 +366,7 @@ template <int P, int BM, int BN, int BK>
 constexpr bool sw_built() {
   using TR = tccg::PrecTraits<P>;
   return std::is_same<typename TR::TG, float>::value && BM % 64 == 0 && BN % 4 == 0 &&
-         sw_lds_bytes(TR::PLANES, (int)sizeof(typename TR::TL), BM, BN, BK, sw_stages<P>()) <= kMaxLds;
+         sw_stages<P, BM, BN, BK>() > 0;
 }
 
 template <int P, int I = 0>
@@ -371,7 +382,7 @@ bool sw_available(int tile) {
 
 template <int P, int BM, int BN, int BK>
 int launch_packed_sw_kernel(const tccg::GettParams<float>& prm, int grid, hipStream_t stream) {
-  constexpr int ST = sw_stages<P>();
+  constexpr int ST = sw_stages<P, BM, BN, BK>();
   using SW = tccg::SwShape<P, BM, BN, BK, ST>;
   auto kern = tccg::gett_packed_sw_kernel<P, BM, BN, BK, ST>;
   static std::atomic<uint64_t> lds_set{0};
@@ -526,22 +537,24 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   prm.crunM = plan->crun_m;
   prm.crunN = plan->crun_n;
   prm.cvec = plan->cvec;
-  if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
-    return fail(1, "crun_m and crun_n must both be positive or both zero");
-  if (plan->crun_m > 0 && !aligned16(C)) return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
   if (plan->walk == 4) {
     if (plan->grid <= 0) return fail(1, "the store-wave kernel (walk 4) needs grid > 0");
-    if (plan->crun_m > 0) return fail(1, "the store-wave kernel (walk 4) has no run epilogue (crun_m must be 0)");
     if (plan->cvec != 0 && plan->cvec != 1) return fail(1, "cvec must be 0 or 1");
-    if (plan->cvec && !aligned16(C)) return fail(1, "cvec == 1 needs a 16-byte aligned C");
+    if (plan->crun_m < 0 || plan->crun_m % 4 || (plan->crun_m > 0 && tc.BM % plan->crun_m))
+      return fail(1, "walk 4: crun_m (the run length of the run store order) must be 0 or a multiple of 4 "
+                     "that divides the tile's BM");
+    if ((plan->cvec || plan->crun_m) && !aligned16(C)) return fail(1, "16-byte stores need a 16-byte aligned C");
     if constexpr (std::is_same<TG, float>::value) {
       return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
     }
     return fail(1, "the store-wave kernel (walk 4) writes FP32 results only");
   }
   if (plan->walk != 0) return fail(1, "packed plans take walk 0 or 4");
+  if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
+    return fail(1, "crun_m and crun_n must both be positive or both zero");
+  if (plan->crun_m > 0 && !aligned16(C)) return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
 }
 

@@ -13,9 +13,9 @@
 //       an LDS staging area; they issue no global stores, so their vmcnt
 //       holds copies only;
 //   waves 4-7 (store): read the staged tile and the tile's C offsets from
-//       LDS and write C with 16-byte stores along C's stride-one index
-//       (beta != 0: C is read and updated); they issue no copies and never
-//       wait for their stores.
+//       LDS and write C with 16-byte stores along C's stride-one index, row
+//       by row or run by run (store_staged) (beta != 0: C is read and
+//       updated); they issue no copies and never wait for their stores.
 //
 // The C offsets of each tile (offCm[m0..m0+BM), offCn[n0..n0+BN)) are copied
 // into LDS by compute wave 0 together with the tile's first k-step, into one
@@ -24,11 +24,11 @@
 //   per tile i: one barrier per k-step, an extra one when KT == 1, then the
 //   barrier that publishes staged tile i.
 //
-// Store waves read staged tile i - 1 (and its offsets) between the first
-// and the second barrier of tile i; compute waves write staged tile i after
-// the second barrier of tile i at the earliest, so one staging area is
-// enough, and a store wave waits only for its LDS reads (lgkmcnt) before a
-// barrier.  The copy cursor runs at most S steps, hence S tiles, ahead of
+// Store waves read staged tile i - 1 (and its offsets) in the intervals
+// after the barriers of k-steps 0 .. KT-2 of tile i (after that of k-step 0
+// only when KT == 1); compute waves write staged tile i after the barrier of
+// k-step KT-1 (the extra one when KT == 1), so one staging area is enough,
+// and a store wave waits only for its LDS reads (lgkmcnt) before a barrier.  The copy cursor runs at most S steps, hence S tiles, ahead of
 // the MFMAs, and the offsets of tile i are read during tile i + 1: NBUF =
 // S + 2 buffers are never overwritten while read.  Persistent: workgroup b
 // walks tiles b, b + gridDim.x, ... in the XCD-chunked, grouped order.
@@ -80,48 +80,109 @@ __device__ inline void copy_offsets(const uint32_t* offCm, const uint32_t* offCn
 }
 
 // Store waves: write staged tile L (rows n of BM floats, alpha applied) to C
-// with the tile's offsets om (BM entries) / on (BN entries).  CVEC: every
-// aligned group of four m positions is four consecutive elements of C
-// starting at a 16-byte boundary (or invalid as a whole).  BETA == false:
-// C is not read and the stores are non-temporal.
-template <int BM, int BN, bool CVEC, bool BETA>
-__device__ inline void store_staged_as(float* C, float beta, const float* L, const uint32_t* om, const uint32_t* on,
-                                       int ts) {
+// with the tile's offsets om (BM entries) / on (BN entries), element by
+// element (C without 16-byte groups).  BETA == false: C is not read and the
+// stores are non-temporal.
+template <int BM, int BN, bool BETA>
+__device__ inline void store_staged_scalar(float* C, float beta, const float* L, const uint32_t* om,
+                                           const uint32_t* on, int ts, int it0, int it1) {
   constexpr int LD = BM + 4, QM = BM / 4;
+  const int e1 = min(BN * QM, 256 * it1);
 #pragma unroll 4
-  for (int e = ts; e < BN * QM; e += 256) {
+  for (int e = ts + 256 * it0; e < e1; e += 256) {
     const int n = e / QM, mq = (e - n * QM) * 4;
     const uint32_t cn = on[n];
     if (cn == kInvalid) continue;
     const f32x4 v = *reinterpret_cast<const f32x4*>(L + n * LD + mq);
-    if constexpr (CVEC) {
-      const uint32_t cm = om[mq];
-      if (cm == kInvalid) continue;
-      f32x4* g = reinterpret_cast<f32x4*>(C + (size_t)(cm + cn));
-      if constexpr (BETA) *g = v + beta * (*g);
-      else __builtin_nontemporal_store(v, g);
-    } else {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t cm = om[mq + j];
-        if (cm == kInvalid) continue;
-        float* g = C + (size_t)(cm + cn);
-        if constexpr (BETA) *g = v[j] + beta * (*g);
-        else __builtin_nontemporal_store(v[j], g);
-      }
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t cm = om[mq + j];
+      if (cm == kInvalid) continue;
+      float* g = C + (size_t)(cm + cn);
+      if constexpr (BETA) *g = v[j] + beta * (*g);
+      else __builtin_nontemporal_store(v[j], g);
     }
   }
 }
 
+// The order in which a store-wave thread writes its 16-byte groups of a
+// staged tile, the same for every tile, so it is computed once per launch.
+// Iteration it covers group e = ts + 256 * it.
+//   Row order (R == 0): e runs along a row of BM positions, then to the next
+//     n row; C must hold every aligned group of four m positions as four
+//     consecutive elements at a 16-byte boundary (cvec).
+//   Run order (R > 0): C's stride-one index spans runs of R consecutive m
+//     positions (R % 4 == 0, BM % R == 0; offCm[m0 + j] = offCm[m0] + j inside
+//     a run, or the whole run invalid) and consecutive n rows of a run mostly
+//     follow each other in memory (C's next index is in the n set): e runs
+//     through one run's groups, then the next n row of the same run, so one
+//     store instruction writes up to a kilobyte of consecutive C even when
+//     runs are short (C[a,b,...] with a in m, b in n: R = extent of a).
+// Per iteration: the n row and the m position whose offset starts the
+// group's run (row order: the group's own), the group's offset inside the
+// run and its position in the staged tile, two 16-bit fields per word
+// (kInvalid: no group).
 template <int BM, int BN>
-__device__ inline void store_staged(const GettParams<float>& p, const float* L, const uint32_t* om,
-                                    const uint32_t* on, int ts) {
-  if (p.cvec) {
-    if (p.beta == 0.f) store_staged_as<BM, BN, true, false>(p.C, 0.f, L, om, on, ts);
-    else store_staged_as<BM, BN, true, true>(p.C, p.beta, L, om, on, ts);
+struct StoreMap {
+  static constexpr int NIT = (BN * (BM / 4) + 255) / 256;
+  static_assert(BM < 65536 && BN * (BM + 4) < 65536, "16-bit fields");
+  uint32_t rowrun[NIT];   // n | run start << 16
+  uint32_t place[NIT];    // staged position | offset in the run << 16
+
+  __device__ inline void init(int ts, int R) {
+    constexpr int LD = BM + 4, QM = BM / 4;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+      const int e = ts + 256 * it;
+      int n, ms, m;
+      if (R > 0) {                       // run order
+        const int VR = R >> 2, per_run = BN * VR;
+        const int mo = e / per_run, rem = e - mo * per_run;
+        n = rem / VR;
+        ms = mo * R;
+        m = ms + (rem - n * VR) * 4;
+      } else {                           // row order
+        n = e / QM;
+        m = (e - n * QM) * 4;
+        ms = m;
+      }
+      rowrun[it] = e < BN * QM ? (uint32_t)n | ((uint32_t)ms << 16) : kInvalid;
+      place[it] = (uint32_t)(n * LD + m) | ((uint32_t)(m - ms) << 16);
+    }
+  }
+};
+
+// 16-byte groups (cvec or run order) with the precomputed map: the group of
+// iteration it goes to offCm[run start] + offset in the run + offCn[n].
+template <int BM, int BN, bool BETA>
+__device__ inline void store_staged_mapped(float* C, float beta, const StoreMap<BM, BN>& map, const float* L,
+                                           const uint32_t* om, const uint32_t* on, int it0, int it1) {
+#pragma unroll
+  for (int it = 0; it < StoreMap<BM, BN>::NIT; ++it) {
+    if (it < it0 || it >= it1) continue;
+    const uint32_t rr = map.rowrun[it];
+    if (rr == kInvalid) continue;
+    const uint32_t cn = on[rr & 0xffffu], cm = om[rr >> 16];
+    if (cn == kInvalid || cm == kInvalid) continue;
+    const uint32_t pl = map.place[it];
+    const f32x4 v = *reinterpret_cast<const f32x4*>(L + (pl & 0xffffu));
+    f32x4* g = reinterpret_cast<f32x4*>(C + (size_t)(cm + (pl >> 16) + cn));
+    if constexpr (BETA) *g = v + beta * (*g);
+    else __builtin_nontemporal_store(v, g);
+  }
+}
+
+// Iterations [it0, it1) of a store wave's share of one staged tile
+// (iteration it covers the 16-byte groups e = ts + 256 * it).
+template <int BM, int BN>
+__device__ inline void store_staged(const GettParams<float>& p, const StoreMap<BM, BN>& map, const float* L,
+                                    const uint32_t* om, const uint32_t* on, int ts, int it0, int it1) {
+  if (p.crunM > 0 || p.cvec) {
+    if (p.beta == 0.f) store_staged_mapped<BM, BN, false>(p.C, 0.f, map, L, om, on, it0, it1);
+    else store_staged_mapped<BM, BN, true>(p.C, p.beta, map, L, om, on, it0, it1);
   } else {
-    if (p.beta == 0.f) store_staged_as<BM, BN, false, false>(p.C, 0.f, L, om, on, ts);
-    else store_staged_as<BM, BN, false, true>(p.C, p.beta, L, om, on, ts);
+    if (p.beta == 0.f) store_staged_scalar<BM, BN, false>(p.C, 0.f, L, om, on, ts, it0, it1);
+    else store_staged_scalar<BM, BN, true>(p.C, p.beta, L, om, on, ts, it0, it1);
   }
 }
 
@@ -153,21 +214,29 @@ gett_packed_sw_kernel(GettParams<float> p) {
 
   if (w >= 4) {
     // ---- store waves ----
+    // The previous staged tile is written in the intervals after the
+    // barriers of k-steps 0 .. KT-2 of the current tile (a share per
+    // interval), so that no barrier waits for all of its stores to issue.
     const int ts = t - 256;
+    StoreMap<BM, BN> map;
+    map.init(ts, p.crunM);
+    constexpr int NIT = StoreMap<BM, BN>::NIT;            // iterations per staged tile
+    const int chunks = KT > 1 ? KT - 1 : 1;
+    const int per = (NIT + chunks - 1) / chunks;
     if (DB) lds_sync();   // the compute waves' prologue barrier
     int v = blockIdx.x, i = 0, vprev = -1;
     for (; v < nblocks; v += gridDim.x, ++i) {
+      const uint32_t* ob = offbuf + ((i + SW::NBUF - 1) % SW::NBUF) * OFFW;   // tile i - 1's offsets
       lds_sync();          // first k-step of tile i
-      if (vprev >= 0) {
-        const uint32_t* ob = offbuf + ((i - 1) % SW::NBUF) * OFFW;
-        store_staged<BM, BN>(p, staged, ob, ob + BM, ts);
+      for (int b = 0; b < after_first; ++b) {
+        if (vprev >= 0 && b < chunks) store_staged<BM, BN>(p, map, staged, ob, ob + BM, ts, b * per, (b + 1) * per);
+        lds_sync();
       }
-      for (int b = 0; b < after_first; ++b) lds_sync();
       vprev = v;
     }
     if (vprev >= 0) {
       const uint32_t* ob = offbuf + ((i - 1) % SW::NBUF) * OFFW;
-      store_staged<BM, BN>(p, staged, ob, ob + BM, ts);
+      store_staged<BM, BN>(p, map, staged, ob, ob + BM, ts, 0, NIT);
     }
     return;
   }

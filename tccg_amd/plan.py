@@ -48,11 +48,13 @@ TILES: Dict[int, List[Tuple[int, int, int]]] = {
 # Tile shapes of the packed path (operands converted once into tile-major
 # planes by the pack kernel); must match csrc/gett_launch.hpp's packed tables.
 PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
-    F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16), (128, 64, 16), (64, 128, 16)],
+    F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16), (128, 64, 16), (64, 128, 16),
+          (192, 64, 16)],
     F64: [(128, 128, 16), (64, 64, 16)],
-    BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 192, 16)],
-    FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128), (192, 192, 128)],
-    BF16: [(128, 128, 32), (256, 128, 32), (128, 256, 32), (64, 64, 32), (192, 192, 32)],
+    BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 192, 16),
+             (192, 64, 16)],
+    FP8: [(128, 128, 128), (256, 128, 128), (64, 64, 128), (192, 192, 128), (192, 64, 128)],
+    BF16: [(128, 128, 32), (256, 128, 32), (128, 256, 32), (64, 64, 32), (192, 192, 32), (192, 64, 32)],
 }
 # Tiles of the vector micro-kernel (walk 3) for short k ranges, FP32 and FP64:
 # BM m positions x BN n positions, BK = the whole padded k range
@@ -85,13 +87,27 @@ def packed_ring_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
     return 3 * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec]
 
 
-def sw_lds_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
-    """LDS of the store-wave kernel (sw_lds_bytes in gett_launch.hpp): the
-    copy ring (2 stages for FP8, else 3), the staged FP32 tile (BN rows of
-    BM + 4 floats) and S + 2 kilobyte-rounded buffers of the tile's C offsets."""
-    S = 2 if prec == FP8 else 3
+def _sw_lds(prec: int, BM: int, BN: int, BK: int, S: int) -> int:
     return (S * _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec] + BN * (BM + 4) * 4
             + (-(-(BM + BN) * 4 // 1024)) * 1024 * (S + 2))
+
+
+def sw_stages(prec: int, BM: int, BN: int, BK: int) -> int:
+    """Ring stages of the store-wave kernel (sw_stages in gett_launch.hpp):
+    the deepest ring of at most 4 stages (at least 3, FP8 2) that fits LDS
+    beside the staged tile and the offset buffers; 0 if none fits."""
+    per_wave = _PLANES[prec] * (BM + BN) * BK * _ELEM_BYTES[prec] // 1024 // 4
+    for S in range(4, (2 if prec == FP8 else 3) - 1, -1):
+        if _sw_lds(prec, BM, BN, BK, S) <= 160 * 1024 and S * per_wave < 64:
+            return S
+    return 0
+
+
+def sw_lds_bytes(prec: int, BM: int, BN: int, BK: int) -> int:
+    """LDS of the store-wave kernel (sw_lds_bytes in gett_launch.hpp): the
+    copy ring of sw_stages() stages, the staged FP32 tile (BN rows of BM + 4
+    floats) and S + 2 kilobyte-rounded buffers of the tile's C offsets."""
+    return _sw_lds(prec, BM, BN, BK, max(2, sw_stages(prec, BM, BN, BK)))
 
 
 def sw_built(prec: int, tile: int) -> bool:
@@ -100,7 +116,7 @@ def sw_built(prec: int, tile: int) -> bool:
     if prec == F64:
         return False
     BM, BN, BK = PACKED_TILES[prec][tile]
-    return BM % 64 == 0 and BN % 4 == 0 and sw_lds_bytes(prec, BM, BN, BK) <= 160 * 1024
+    return BM % 64 == 0 and BN % 4 == 0 and sw_stages(prec, BM, BN, BK) > 0
 
 
 class UnsupportedContraction(ValueError):
@@ -361,12 +377,13 @@ class GettPlan:
         if self.walk == 3:
             walk = " vector"
         elif self.walk == 4:
-            walk = " storewaves grid=%d%s" % (self.grid, " cvec" if self.cvec else "")
+            walk = " storewaves grid=%d%s%s" % (self.grid, " cvec" if self.cvec else "",
+                                               " runs=%d" % self.crun[0] if self.crun[0] else "")
         elif self.walk == 2:
             walk = " stream grid=%d" % self.grid
         elif self.grid:
             walk = " persistent grid=%d" % self.grid
-        if self.crun[0]:
+        if self.crun[0] and self.walk != 4:
             walk += " runs=%dx%d" % self.crun
         return ("prec=%s tile=%d (%dx%dx%d)%s%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
                 % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, " packed" if self.packed else "",
@@ -417,6 +434,36 @@ def _cvec_ok(offCm: np.ndarray, offCn: np.ndarray) -> bool:
     return _vector_ok_free(offCm, np.arange(offCm.size), offCm.size) and bool(np.all(cn % 4 == 0))
 
 
+def _store_run(offCm: np.ndarray, offCn: np.ndarray, BM: int) -> int:
+    """Run length R of the store-wave kernel's run order, or 0 for its row
+    order: R is the length of C's contiguous m runs (from the first m
+    position) when R is a multiple of 4 below BM that divides BM, every
+    aligned run of R positions is R consecutive offsets from a multiple of 4
+    (or invalid as a whole), every valid n offset is a multiple of 4 and at
+    least half of the consecutive n rows follow each other in memory (offCn
+    steps by R): then a run's rows form one contiguous stretch of C."""
+    cm = offCm.astype(np.int64)
+    if cm.size == 0 or cm[0] == np.int64(INVALID):
+        return 0
+    R = 1
+    while R < cm.size and cm[R] == cm[0] + R:
+        R += 1
+    if R % 4 or R >= BM or BM % R or cm.size % R:
+        return 0
+    t = cm.reshape(-1, R)
+    inv = t == np.int64(INVALID)
+    if np.any(inv.any(axis=1) & ~inv.all(axis=1)):
+        return 0
+    t = t[~inv.all(axis=1)]
+    if not (np.all(t[:, 0] % 4 == 0) and np.all(t - t[:, :1] == np.arange(R))):
+        return 0
+    cn = offCn.astype(np.int64)
+    cn = cn[cn != np.int64(INVALID)]
+    if not np.all(cn % 4 == 0) or cn.size < 2:
+        return 0
+    return R if float(np.mean(np.diff(cn) == R)) >= 0.5 else 0
+
+
 def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
                       persistent: bool = False, store_waves: bool = False) -> GettPlan:
     """Plan of the packed path: the reference's pack-then-macro-kernel GETT
@@ -452,6 +499,7 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
         plan.walk = 4
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
         plan.cvec = int(_cvec_ok(offCm, offCn))
+        plan.crun = (_store_run(offCm, offCn, BM), 0)
         return plan
     if not persistent:
         esz = 8 if prec == F64 else 4
@@ -722,7 +770,7 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     if plan.walk == 4:
         # store waves write C while the compute waves run the next tile: the
         # slower of the two per tile (16-byte groups store at a higher rate)
-        t_sw = esz * plan.BM * plan.BN / ((0.8 if plan.cvec else 0.5) * hw["hbm_bytes_per_s"] / slots)
+        t_sw = esz * plan.BM * plan.BN / ((0.8 if (plan.cvec or plan.crun[0]) else 0.5) * hw["hbm_bytes_per_s"] / slots)
         t = rounds * (max(t_mfma, t_sw) + 0.3e-6) + t_pack
     else:
         t = rounds * (t_mfma + t_fixed) + t_pack

@@ -63,22 +63,28 @@ def compute_role(KT, ntiles, S, db, nbuf):
 
 
 def store_role(KT, ntiles, db):
+    """reads[i] = (first, last) interval in which staged tile i is read: the
+    store waves spread a tile over the intervals after the barriers of k-steps
+    0 .. KT-2 of the next tile (one interval when KT == 1)."""
     b = 0
     reads = {}
     after_first = (KT - 1) + (1 if KT == 1 else 0) + 1
+    chunks = KT - 1 if KT > 1 else 1
     if db:
         b += 1
     for i in range(ntiles):
         b += 1
-        if i > 0:
-            reads[i - 1] = b
-        b += after_first
+        first = b
+        for j in range(after_first):
+            if i > 0 and j < chunks:
+                reads[i - 1] = (first, b)
+            b += 1
     if ntiles:
-        reads[ntiles - 1] = b
+        reads[ntiles - 1] = (b, b)
     return b, reads
 
 
-@pytest.mark.parametrize("S,db", [(3, True), (4, True), (2, False), (3, False)])
+@pytest.mark.parametrize("S,db", [(3, True), (4, True), (2, False), (3, False), (4, False)])
 def test_store_wave_protocol(S, db):
     nbuf = S + 2
     for KT, ntiles in itertools.product(range(1, 7), range(1, 10)):
@@ -86,18 +92,18 @@ def test_store_wave_protocol(S, db):
         nb_s, reads = store_role(KT, ntiles, db)
         assert nb_c == nb_s, (KT, ntiles)
         for i in range(ntiles):
-            assert reads[i] > writes[i], ("read before publish", KT, ntiles, i)
+            assert reads[i][0] > writes[i], ("read before publish", KT, ntiles, i)
             if i + 1 < ntiles:
-                assert writes[i + 1] > reads[i], ("overwrite while read", KT, ntiles, i)
+                assert writes[i + 1] > reads[i][1], ("overwrite while read", KT, ntiles, i)
         # offsets of tile i: the issue call that copied them
         dma = {off: (call, iv) for call, (iv, off) in enumerate(issued) if off is not None}
         assert sorted(dma) == list(range(ntiles))
         for i in range(ntiles):
             call, iv = dma[i]
             # a barrier before the read interval waited for this call or a later one
-            assert any(bar < reads[i] and aw >= call for bar, aw in waits), ("offsets not landed", KT, ntiles, i)
+            assert any(bar < reads[i][0] and aw >= call for bar, aw in waits), ("offsets not landed", KT, ntiles, i)
             if i + nbuf in dma:
-                assert dma[i + nbuf][1] > reads[i], ("offset buffer reused while read", KT, ntiles, i)
+                assert dma[i + nbuf][1] > reads[i][1], ("offset buffer reused while read", KT, ntiles, i)
 
 
 def test_model_catches_a_missing_extra_barrier():
@@ -112,4 +118,4 @@ def test_model_catches_a_missing_extra_barrier():
         writes[i] = b
         b += 1
     nb_s, reads = store_role(KT, ntiles, True)
-    assert any(writes[i + 1] <= reads[i] for i in range(ntiles - 1)) or b != nb_s
+    assert any(writes[i + 1] <= reads[i][1] for i in range(ntiles - 1)) or b != nb_s

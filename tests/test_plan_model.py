@@ -71,13 +71,30 @@ def test_every_plan_addresses_the_contraction(seed):
 
 
 def emulate_store_waves(plan, C_exact, C_init):
-    """The store-wave kernel's write of a finished tile: with cvec, aligned
-    groups of four m positions are written as four consecutive elements from
-    offCm[m4] + offCn[n] (16-byte aligned); without, element by element."""
+    """The store-wave kernel's write of a finished tile: in run order (crun
+    R > 0) the group of four m positions m4 of a run starting at r0 goes to
+    offCm[r0] + (m4 - r0) + offCn[n]; with cvec, aligned groups of four m
+    positions are written as four consecutive elements from offCm[m4] +
+    offCn[n] (16-byte aligned); otherwise element by element."""
     cm = plan.offCm.astype(np.int64)
     cn = plan.offCn.astype(np.int64)
     out = C_init.reshape(-1).copy()
     exact = C_exact.reshape(-1)
+    R = plan.crun[0]
+    if R:
+        assert plan.BM % R == 0 and R % 4 == 0
+        for n in range(cn.size):
+            if cn[n] == INV:
+                continue
+            for r0 in range(0, cm.size, R):
+                if cm[r0] == INV:
+                    continue
+                for m4 in range(r0, r0 + R, 4):
+                    base = cm[r0] + (m4 - r0) + cn[n]
+                    assert base % 4 == 0
+                    for j in range(4):
+                        out[base + j] = exact[cm[m4 + j] + cn[n]]
+        return out.reshape(C_init.shape)
     for n in range(cn.size):
         if cn[n] == INV:
             continue
@@ -123,3 +140,30 @@ def test_store_wave_groups(seed):
                 assert plan.cvec, plan.describe()
             out = emulate_store_waves(plan, exact, init)
             assert np.array_equal(out, exact), (str(c), plan.describe())
+
+
+def test_store_wave_run_order():
+    """C[a,b,...] with a in the m set and b in the n set: the run order is
+    chosen (runs of a's extent) when a's extent divides BM, and its writes
+    land on the right elements; the precision contraction's shape gets runs
+    of 24 with the 192-row tiles."""
+    from tccg_amd.plan import FP8, PACKED_TILES, build_packed_plan, sw_built
+    rng = np.random.default_rng(7)
+    c = Contraction.from_labels(["a", "b", "c", "d"], ["d", "k", "a"], ["k", "c", "b"],
+                                dict(a=8, b=5, c=3, d=7, k=4))
+    sc = c.shapes()[2]
+    exact = rng.uniform(-1, 1, sc)
+    seen = 0
+    for prec in (F32, FP8):
+        for tile in range(len(PACKED_TILES[prec])):
+            if not sw_built(prec, tile):
+                continue
+            plan = build_packed_plan(c.problem, prec, tile, store_waves=True)
+            assert plan.crun[0] == (8 if plan.BM % 8 == 0 else 0), plan.describe()
+            assert np.array_equal(emulate_store_waves(plan, exact, np.full(sc, np.nan)), exact)
+            seen += 1
+    assert seen
+    from tccg_amd.workloads import WORKLOADS
+    c3 = Contraction.from_tccg(WORKLOADS["abcdef-dgfa-gceb"])
+    t = PACKED_TILES[FP8].index((192, 64, 128))
+    assert build_packed_plan(c3.problem, FP8, t, store_waves=True).crun[0] == 24
