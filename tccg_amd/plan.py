@@ -714,6 +714,34 @@ def estimate_seconds(plan: GettPlan, beta: float = 0.0) -> float:
     return _estimate(plan, beta)[0]
 
 
+def store_efficiency(segment_bytes: float) -> float:
+    """Fraction of HBM bandwidth that C's write-back reaches when each store
+    instruction writes contiguous segments of this many bytes (measured on
+    MI355X: kilobyte segments ~0.7, 96-byte ones ~0.35)."""
+    return 0.75 * min(1.0, max(segment_bytes, 16.0) / 1024.0) ** 0.3
+
+
+def _c_segment_bytes(plan: GettPlan) -> float:
+    """Contiguous bytes of C a store instruction of the plan's epilogue writes."""
+    esz = 8 if plan.prec == F64 else 4
+    cm = plan.offCm.astype(np.int64)
+    L = 1
+    while L < min(cm.size, 4096) and cm[L] == cm[0] + L:
+        L += 1
+    if plan.walk == 4:
+        if plan.crun[0]:
+            cn = plan.offCn.astype(np.int64)[:plan.BN]
+            adj = 1
+            while adj < cn.size and cn[adj] == cn[adj - 1] + plan.crun[0]:
+                adj += 1
+            return min(1024.0, plan.crun[0] * adj * esz)
+        return min(L, plan.BM) * esz if plan.cvec else esz
+    if plan.packed and plan.crun[0] and not plan.grid:
+        return min(1024.0, plan.crun[0] * plan.crun[1] * esz)
+    sub = 16 if plan.prec in (FP8, F64) else 32   # MFMA output columns along m
+    return min(L, sub) * esz
+
+
 def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     """Estimate of one launch, the gfx950 analogue of Gett.estimateGFLOPS
     (tccg/gett.py:590-687): per tile, MFMA time of the padded work plus a fixed
@@ -748,7 +776,9 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     if plan.packed:
         # contiguous tile copies, no per-tile conversion; the pack kernels read
         # each operand once and write its planes once
-        eff = 0.85 if plan.BM * plan.BN >= 128 * 128 else 0.6
+        eff = 0.88 if plan.BM * plan.BN >= 192 * 192 else 0.85 if plan.BM * plan.BN >= 128 * 128 else 0.6
+        if plan.walk == 4:
+            eff *= 0.92   # the staged tile's LDS writes and the store waves' issue slots
         src = 8 if plan.prec == F64 else 4
         moved = (src + _PLANES[plan.prec] * _ELEM_BYTES[plan.prec]) * (plan.Mp + plan.Np) * plan.Kp
         t_pack = 6.0e-6 + moved / (0.6 * hw["hbm_bytes_per_s"])
@@ -760,7 +790,19 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
         # operands split or quantised again for every tile that reads them
         eff *= {BF16X3: 0.45, BF16: 0.6, FP8: 0.6}.get(plan.prec, 1.0)
     t_mfma = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots) / eff
-    t_store = esz * plan.BM * plan.BN / (0.7 * hw["hbm_bytes_per_s"] / slots)
+    if plan.packed:
+        # LDS traffic per tile: every k-step's copy into the ring plus the four
+        # waves' fragment reads (each wave reads its half of A's and B's rows),
+        # and for the store-wave kernel the staged tile written and read once;
+        # 128 bytes per clock per CU
+        ring = 3.0 * _PLANES[plan.prec] * _ELEM_BYTES[plan.prec] * (plan.BM + plan.BN) * plan.Kp
+        staged = 2.0 * 4 * plan.BM * plan.BN if plan.walk == 4 else 0.0
+        t_lds = (ring + staged) / (128 * hw["clock_ghz"] * 1e9) * (hw["cus"] / slots)
+        # measured: the compute waves' LDS traffic adds to their MFMA time when
+        # it is comparable (short k-steps of small wave tiles)
+        t_mfma = max(t_mfma, t_lds) + (min(t_mfma, t_lds) if plan.walk == 4 else 0.0)
+    seff = store_efficiency(_c_segment_bytes(plan))
+    t_store = esz * plan.BM * plan.BN / (seff * hw["hbm_bytes_per_s"] / slots)
     t_fixed = 2.0e-6 + t_store
     if plan.walk == 2:
         t_fixed = 0.1e-6 + t_store * 0.5
@@ -769,9 +811,8 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
     if plan.walk == 4:
         # store waves write C while the compute waves run the next tile: the
-        # slower of the two per tile (16-byte groups store at a higher rate)
-        t_sw = esz * plan.BM * plan.BN / ((0.8 if (plan.cvec or plan.crun[0]) else 0.5) * hw["hbm_bytes_per_s"] / slots)
-        t = rounds * (max(t_mfma, t_sw) + 0.3e-6) + t_pack
+        # slower of the two per tile, plus staging the tile through LDS
+        t = rounds * (max(t_mfma, t_store) + 0.3e-6) + t_pack
     else:
         t = rounds * (t_mfma + t_fixed) + t_pack
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
