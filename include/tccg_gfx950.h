@@ -190,6 +190,22 @@ int tccg_gett_packed_bf16(const float* A, const float* B, float* C, float alpha,
                           const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
 int tccg_gett_packed_fp8(const float* A, const float* B, float* C, float alpha, float beta,
                          const tccg_gett_plan* plan, void* workspace, size_t workspace_bytes, void* stream);
+/*
+ * Packed GETT without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same as
+ * tccg_gett_packed_<prec> for precision `prec` (A, B, C of its type, alpha
+ * and beta converted to it), with the workspace in a device array of the
+ * library of TCCG_NOALLOC_SCRATCH_BYTES (128 MiB) that every GPU holds for
+ * the life of the process; plans whose packed operands need more
+ * (tccg_packed_workspace_bytes()) are rejected.  Calls never allocate.  All
+ * calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (several streams or host threads) race on it.
+ */
+#define TCCG_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int tccg_gett_packed_noalloc(int prec, const void* A, const void* B, void* C, double alpha, double beta,
+                             const tccg_gett_plan* plan, void* stream);
+
 /* The GETT kernel alone over operands that a previous tccg_gett_packed_<prec>
  * call with the same plan and workspace has packed (A and B unchanged since):
  * for callers that reuse packed operands across calls, and for timing the
