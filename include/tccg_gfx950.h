@@ -128,6 +128,13 @@ typedef struct tccg_gett_plan {
                        /* TCCG_INVALID) and every offCn entry is a multiple */
                        /* of 4: C is then written with 16-byte stores (C    */
                        /* must be 16-byte aligned); 0 otherwise             */
+  int32_t ksplit;      /* packed plans, walk 0, grid 0: when > 1, each C    */
+                       /* tile's k range is split into ksplit contiguous    */
+                       /* parts computed by separate workgroups into        */
+                       /* partial tiles in the workspace, then added in a   */
+                       /* fixed order and written with alpha/beta (the      */
+                       /* reference's parallel-K, tccg/gett.py:755-781);    */
+                       /* at most Kp / BK; 0 or 1: no split                 */
 } tccg_gett_plan;
 
 /* Tile shape (BM x BN x BK) of tile configuration `tile` for precision `prec`. */

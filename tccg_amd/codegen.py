@@ -54,6 +54,27 @@ def _table_names(name: str):
     return {k: "%s_%s" % (base, k) for k in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB")}
 
 
+def _part_expr(plan: GettPlan, T: str) -> str:
+    """The partial-tile buffer of a split-k plan: the workspace after the
+    packed operands, at a 256-byte boundary (tccg_packed_workspace_bytes)."""
+    if plan.ksplit <= 1:
+        return "nullptr"
+    per = {F32: 4, F64: 8, BF16X3: 6, BF16: 2, FP8: 1}[plan.prec] * plan.Kp
+    ab = -(-(per * plan.Mp) // 256) * 256 + per * plan.Np
+    return "reinterpret_cast<%s*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % (T, -(-ab // 256) * 256)
+
+
+def _reduce_launch(plan: GettPlan, T: str) -> str:
+    if plan.ksplit <= 1:
+        return ""
+    groups = plan.ntiles() * plan.BN * (plan.BM // 4)
+    grid = min(-(-groups // 256), 2048)
+    return ("""  hipLaunchKernelGGL((tccg::split_reduce_kernel<%s, %d, %d>), dim3(%d), dim3(256), 0, s, p);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("split reduction launch: ") + hipGetErrorString(e));
+""" % (T, plan.BM, plan.BN, grid))
+
+
 def emit_source(plan: GettPlan, contraction_text: str, name: str = "gett") -> str:
     """HIP source of the per-contraction library for `plan`."""
     pb = plan.problem
@@ -251,15 +272,18 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
   p.alpha = alpha; p.beta = beta;
   p.amax = %s;
   p.crunM = %d; p.crunN = %d; p.cvec = %d;
+  p.ksplit = %d; p.part = %s;
   hipLaunchKernelGGL(kern, dim3(%d), dim3(%d), kLds, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
-  return 0;
+%s  return 0;
 }
 
 """ % (T, plan.Mp // plan.BM, plan.Np // plan.BN, plan.Kp // plan.BK, plan.groupM,
        "t->amax" if plan.prec == FP8 else "nullptr", plan.crun[0], plan.crun[1], plan.cvec,
-       plan.grid if plan.grid else plan.ntiles(), 512 if plan.walk == 4 else 256))
+       plan.ksplit, _part_expr(plan, T),
+       plan.grid if plan.grid else plan.ntiles() * plan.ksplit, 512 if plan.walk == 4 else 256,
+       _reduce_launch(plan, T)))
     out.append("void gett({T}* A, {T}* B, {T}* C, const {T} alpha, const {T} beta) {{\n"
                "  (void)gett_stream(A, B, C, alpha, beta, nullptr);\n}}\n\n".format(T=T))
     # sizes of the .tccg's A, B and C (gett()'s arguments in their positional order)
@@ -324,6 +348,25 @@ def workload_library_name(workload: str, prec: int) -> str:
     return "gett_%s_%s" % (workload.replace("-", "_"), PREC_NAMES[prec])
 
 
+def packed_suffix(packed) -> str:
+    """Library-name suffix of a packed plan's options (none for planner plans)."""
+    if packed is None:
+        return ""
+    if packed.get("store_waves"):
+        return "_storewaves"
+    if packed.get("ksplit", 1) > 1:
+        return "_ksplit%d" % packed["ksplit"]
+    return "_packed"
+
+
+# Contractions built into libraries besides the named workloads: the first
+# adjoint of abcd-aklb-lckd (dA[a,k,l,b] = dC[a,b,c,d] * B[l,c,k,d]) at
+# extents 24/16, M*N = 576*256 with k = 576.
+SPLIT_K_TEXTS = {
+    "aklb-abcd-lckd": "C[a,k,l,b] = A[a,b,c,d] * B[l,c,k,d]\na = 24\nb = 24\nc = 24\nd = 24\nk = 16\nl = 16\n",
+}
+
+
 def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) -> list:
     """Generate and compile the per-contraction libraries of the named
     workloads (used by the GPU tests of the generated entry point)."""
@@ -334,16 +377,22 @@ def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) 
     jobs = []
     names = workloads or ["example", "cpu48", "abcd-aebf-dfce", "abc-akb-kc"]
     todo = [(wl, prec, None) for wl in names for prec in precs]
+    texts = dict(WORKLOADS)
     if workloads is None:
         # packed-operand libraries: split-BF16 on the compute-bound contraction,
-        # FP8 (persistent walk) on the precision contraction
+        # FP8 (persistent walk) on the precision contraction, FP8 store waves,
+        # and a split-k plan on the first adjoint of abcd-aklb-lckd (small M*N,
+        # long k)
+        texts.update(SPLIT_K_TEXTS)
         todo += [("abcd-aebf-dfce", BF16X3, dict(tile=0, persistent=False)),
-                 ("abcdef-dgfa-gceb", FP8, dict(tile=0, persistent=True))]
+                 ("abcdef-dgfa-gceb", FP8, dict(tile=0, persistent=True)),
+                 ("abcdef-dgfa-gceb", FP8, dict(tile=4, store_waves=True)),
+                 ("aklb-abcd-lckd", F32, dict(tile=0, ksplit=4))]
     for wl, prec, packed in todo:
-        text = WORKLOADS[wl]
+        text = texts[wl]
         c = Contraction.from_tccg(text)
         plan = c.plan(prec) if packed is None else build_packed_plan(c.problem, prec, **packed)
-        nm = workload_library_name(wl, prec) + ("_packed" if packed is not None else "")
+        nm = workload_library_name(wl, prec) + packed_suffix(packed)
         src_text = emit_source(plan, text, nm)
         digest = hashlib.sha256((src_text + "".join((_build.CSRC / h).read_text() for h in _build.LIB_HEADERS)).encode()).hexdigest()[:16]
         lib = GENERATED_DIR / ("lib%s.so" % nm)

@@ -88,6 +88,13 @@ struct GettParams {
   // consecutive elements of C at a 16-byte boundary (or invalid as a whole),
   // and C is 16-byte aligned: 16-byte stores.
   int cvec;
+  // Packed kernel, one tile per workgroup: ksplit > 1 splits each tile's k
+  // range into ksplit contiguous parts, one workgroup each, which write
+  // their raw sums (FP8: descaled) to part[(ks * tiles + tile) * BN * BM]
+  // (rows n of BM), tile = tm * tilesN + tn; split_reduce_kernel adds the
+  // parts and applies alpha and beta.
+  int ksplit;
+  TG* part;
 };
 
 template <int P> struct PrecTraits;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <string>
 #include <type_traits>
@@ -269,6 +270,8 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   prm.crunM = 0;
   prm.crunN = 0;
   prm.cvec = 0;
+  prm.ksplit = 1;
+  prm.part = nullptr;
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
@@ -297,9 +300,11 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
 }
 
 
-// Bytes of the packed operands of a plan: A's planes, padded to 256 bytes,
-// then B's planes.  0 if the plan's shape is not one of the packed tiles.
-inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB) {
+// Bytes of a packed plan's workspace: A's planes, padded to 256 bytes, B's
+// planes, and with ksplit > 1 (padded to 256 bytes) the ksplit partial C
+// tiles of every tile (FP64 for FP64, else FP32).  0 if the plan's shape is
+// not one of the packed tiles.
+inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB, size_t* offP = nullptr) {
   int ntiles = 0;
   const TileCfg* tiles = packed_tiles_of(prec, &ntiles);
   if (!tiles || !plan || plan->tile < 0 || plan->tile >= ntiles || plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0)
@@ -315,6 +320,10 @@ inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_
   const size_t a = (planes * (size_t)plan->Mp * (size_t)plan->Kp * esz + 255) & ~(size_t)255;
   const size_t b = planes * (size_t)plan->Np * (size_t)plan->Kp * esz;
   if (offB) *offB = a;
+  const size_t ab = (a + b + 255) & ~(size_t)255;
+  if (offP) *offP = ab;
+  if (plan->ksplit > 1)
+    return ab + (size_t)plan->ksplit * (size_t)plan->Mp * (size_t)plan->Np * (prec == tccg::F64 ? 8 : 4);
   return a + b;
 }
 
@@ -436,6 +445,18 @@ int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTra
         if (grid > 0 && grid < nblocks) return fail(1, "the run epilogue is built for one tile per workgroup");
       }
       if (grid > 0 && grid < nblocks) return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, true>(prm, grid, s);
+      if (prm.ksplit > 1) {
+        // (k part, tile) work items, then the reduction of the parts into C
+        int rc = launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, false>(prm, nblocks * prm.ksplit, s);
+        if (rc) return rc;
+        using TG = typename tccg::PrecTraits<P>::TG;
+        const long groups = (long)nblocks * c.BN * (c.BM / 4);
+        const unsigned rgrid = (unsigned)std::min<long>((groups + 255) / 256, 2048);
+        hipLaunchKernelGGL((tccg::split_reduce_kernel<TG, c.BM, c.BN>), dim3(rgrid), dim3(256), 0, s, prm);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail((int)e, std::string("split reduction launch: ") + hipGetErrorString(e));
+        return 0;
+      }
       return launch_packed_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN, false>(prm, nblocks, s);
     }
     return dispatch_packed_tile<P, I + 1>(tile, prm, grid, s);
@@ -505,21 +526,18 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   if (plan->Mp % tc.BM || plan->Np % tc.BN || plan->Kp % tc.BK)
     return fail(1, "table lengths must be whole multiples of the tile shape");
   if (plan->Kp % tccg::kPackTK) return fail(1, "packed plans need Kp to be a multiple of 32");
-  size_t offB = 0;
-  const size_t need = packed_workspace_bytes(P, plan, &offB);
+  size_t offB = 0, offP = 0;
+  const size_t need = packed_workspace_bytes(P, plan, &offB, &offP);
   if (workspace_bytes < need) return fail(1, "workspace too small for the packed operands");
   // the packed kernels address elements with 32-bit offsets
-  if ((need - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
+  const size_t endB = plan->ksplit > 1 ? offP : need;
+  if ((endB - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
     return fail(1, "packed operand exceeds 2^32-1 elements");
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   if (plan->grid < 0) return fail(1, "grid must be >= 0");
   TL* wA = reinterpret_cast<TL*>(workspace);
   TL* wB = reinterpret_cast<TL*>(reinterpret_cast<unsigned char*>(workspace) + offB);
   hipStream_t s = (hipStream_t)stream;
-  if (pack) {
-    int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
-    if (rc) return rc;
-  }
   tccg::GettParams<TG> prm;
   prm.A = reinterpret_cast<const TG*>(wA);
   prm.B = reinterpret_cast<const TG*>(wB);
@@ -537,8 +555,19 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   prm.crunM = plan->crun_m;
   prm.crunN = plan->crun_n;
   prm.cvec = plan->cvec;
+  prm.ksplit = plan->ksplit > 1 ? plan->ksplit : 1;
+  prm.part = reinterpret_cast<TG*>(reinterpret_cast<unsigned char*>(workspace) + offP);
+  if (plan->ksplit < 0) return fail(1, "ksplit must be >= 0");
+  if (prm.ksplit > 1) {
+    if (plan->walk != 0 || (plan->grid > 0 && plan->grid < (int64_t)prm.tilesM * prm.tilesN))
+      return fail(1, "ksplit > 1 runs one (k part, tile) per workgroup: walk 0 and grid 0");
+    if (plan->crun_m > 0) return fail(1, "ksplit > 1 has no run epilogue (crun_m must be 0)");
+    if (prm.ksplit > prm.kTiles) return fail(1, "ksplit exceeds the k-steps of the plan");
+    if ((int64_t)prm.tilesM * prm.tilesN * prm.ksplit > 0x7fffffff) return fail(1, "too many work items");
+  }
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
+  // every check before the pack kernels: a rejected plan launches nothing
   if (plan->walk == 4) {
     if (plan->grid <= 0) return fail(1, "the store-wave kernel (walk 4) needs grid > 0");
     if (plan->cvec != 0 && plan->cvec != 1) return fail(1, "cvec must be 0 or 1");
@@ -546,15 +575,21 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
       return fail(1, "walk 4: crun_m (the run length of the run store order) must be 0 or a multiple of 4 "
                      "that divides the tile's BM");
     if ((plan->cvec || plan->crun_m) && !aligned16(C)) return fail(1, "16-byte stores need a 16-byte aligned C");
-    if constexpr (std::is_same<TG, float>::value) {
-      return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
-    }
-    return fail(1, "the store-wave kernel (walk 4) writes FP32 results only");
+    if (!std::is_same<TG, float>::value) return fail(1, "the store-wave kernel (walk 4) writes FP32 results only");
+  } else {
+    if (plan->walk != 0) return fail(1, "packed plans take walk 0 or 4");
+    if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
+      return fail(1, "crun_m and crun_n must both be positive or both zero");
+    if (plan->crun_m > 0 && !aligned16(C))
+      return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   }
-  if (plan->walk != 0) return fail(1, "packed plans take walk 0 or 4");
-  if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
-    return fail(1, "crun_m and crun_n must both be positive or both zero");
-  if (plan->crun_m > 0 && !aligned16(C)) return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
+  if (pack) {
+    int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
+    if (rc) return rc;
+  }
+  if (plan->walk == 4) {
+    if constexpr (std::is_same<TG, float>::value) return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
+  }
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
 }
 

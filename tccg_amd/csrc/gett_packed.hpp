@@ -267,25 +267,34 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   using FR = PackedFrags<P, BM, BN, BK, WM, WN>;
   extern __shared__ __align__(1024) unsigned char smem_raw[];
 
-  const int nblocks = p.tilesM * p.tilesN;
+  const int ntiles = p.tilesM * p.tilesN;
+  const int ksplit = PERSIST ? 1 : p.ksplit;
+  const int nblocks = ntiles * ksplit;   // work items: tiles, or (k part, tile) pairs
   int v = blockIdx.x;
   if (v >= nblocks) return;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = w % WM, wn = w / WM;
-  const int KT = p.kTiles;
   const unsigned char* Ab = reinterpret_cast<const unsigned char*>(p.A);
   const unsigned char* Bb = reinterpret_cast<const unsigned char*>(p.B);
 
   int tm, tn;
-  tile_coords(xcd_chunked(v, nblocks), p.tilesM, p.tilesN, p.groupM, tm, tn);
+  int lv = xcd_chunked(v, nblocks), ks = 0;
+  if (ksplit > 1) {
+    ks = lv / ntiles;   // consecutive work items share a k part: neighbouring tiles share its blocks
+    lv -= ks * ntiles;
+  }
+  tile_coords(lv, p.tilesM, p.tilesN, p.groupM, tm, tn);
+  // k-steps [k0, k0 + KT) of the tile (all of them without a split)
+  const int k0 = (int)((long)ks * p.kTiles / ksplit);
+  const int KT = (int)((long)(ks + 1) * p.kTiles / ksplit) - k0;
   // Copy cursor: the next k-step to fetch.  Past the workgroup's last k-step
   // it stays there (a harmless re-read), so every step issues the same
   // number of copies and the vmcnt arithmetic holds.
   int cv = v, ck = 0;
-  const unsigned char* cA = Ab + (size_t)tm * KT * SH::BYTES_A;
-  const unsigned char* cB = Bb + (size_t)tn * KT * SH::BYTES_B;
+  const unsigned char* cA = Ab + ((size_t)tm * p.kTiles + k0) * SH::BYTES_A;
+  const unsigned char* cB = Bb + ((size_t)tn * p.kTiles + k0) * SH::BYTES_B;
   auto issue = [&](int slot) {
     packed_copy<P, BM, BN, BK, S>(cA, cB, smem_raw + slot * SH::STAGE_BYTES, w, lane);
     if (ck + 1 < KT) {
@@ -297,8 +306,8 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       ck = 0;
       int ctm, ctn;
       tile_coords(xcd_chunked(cv, nblocks), p.tilesM, p.tilesN, p.groupM, ctm, ctn);
-      cA = Ab + (size_t)ctm * KT * SH::BYTES_A;
-      cB = Bb + (size_t)ctn * KT * SH::BYTES_B;
+      cA = Ab + (size_t)ctm * p.kTiles * SH::BYTES_A;
+      cB = Bb + (size_t)ctn * p.kTiles * SH::BYTES_B;
     }
   };
 
@@ -387,7 +396,25 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       step(f0, f1);
       if constexpr (DB) f0 = f1;  // the next tile's first fragments
     }
-    if (!PERSIST && p.crunM > 0) {
+    if (!PERSIST && ksplit > 1) {
+      // raw sums of this k part into the partial-tile buffer, rows n of BM
+      using TG = typename PrecTraits<P>::TG;
+      TG* dst = p.part + ((size_t)ks * ntiles + (size_t)tm * p.tilesN + tn) * (BN * BM);
+#pragma unroll
+      for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+        for (int r = 0; r < MM::NACC; ++r) {
+          int n_l, m_l;
+          MM::coord(lane, r, n_l, m_l);
+          const int n = wn * TS::WTN + i * PrecTraits<P>::SUB + n_l;
+#pragma unroll
+          for (int j = 0; j < TS::SM; ++j) {
+            TG val = (TG)acc[i][j][r];
+            if (P == FP8) val = (TG)((float)val * descale);
+            dst[n * BM + wm * TS::WTM + j * PrecTraits<P>::SUB + m_l] = val;
+          }
+        }
+    } else if (!PERSIST && p.crunM > 0) {
       wait_vm_lgkm0<0>();  // the ring is reused for staging: every copy has landed
       __syncthreads();
       store_tile_staged<P, BM, BN, BK, WM, WN>(p, acc, tm * BM, tn * BN, wm, wn, lane, descale, smem_raw,
@@ -407,6 +434,39 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   }
   // No copy may still be writing this workgroup's LDS when it retires.
   wait_vm_lgkm0<0>();
+}
+
+// C = alpha * (sum of the ksplit partial tiles) + beta * C, the partial sums
+// added in k-part order (deterministic).  Thread g handles four consecutive
+// m positions of one row n of one tile.
+template <class TG, int BM, int BN>
+__global__ void __launch_bounds__(256) split_reduce_kernel(GettParams<TG> p) {
+  const int ntiles = p.tilesM * p.tilesN;
+  constexpr int QM = BM / 4;
+  const long total = (long)ntiles * BN * QM;
+  const size_t tile_elems = (size_t)BN * BM, part_elems = (size_t)ntiles * tile_elems;
+  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
+    const int tile = (int)(g / (BN * QM));
+    const int rem = (int)(g - (long)tile * BN * QM);
+    const int n = rem / QM, m4 = (rem - n * QM) * 4;
+    const int tm = tile / p.tilesN, tn = tile - tm * p.tilesN;
+    const uint32_t cn = p.offCn[tn * BN + n];
+    if (cn == kInvalid) continue;
+    const TG* src = p.part + (size_t)tile * tile_elems + (size_t)n * BM + m4;
+    TG s[4] = {TG(0), TG(0), TG(0), TG(0)};
+    for (int ks = 0; ks < p.ksplit; ++ks) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s[j] += src[(size_t)ks * part_elems + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t cm = p.offCm[tm * BM + m4 + j];
+      if (cm == kInvalid) continue;
+      TG* dst = p.C + (size_t)(cm + cn);
+      if (p.beta == TG(0)) __builtin_nontemporal_store(p.alpha * s[j], dst);
+      else *dst = p.alpha * s[j] + p.beta * (*dst);
+    }
+  }
 }
 
 }  // namespace tccg

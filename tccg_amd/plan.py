@@ -343,6 +343,7 @@ class GettPlan:
                            # 4: packed, store-wave kernel over `grid` workgroups
     crun: Tuple[int, int] = (0, 0)  # packed: (E0, E1) blocks of C written as contiguous runs
     cvec: int = 0          # store-wave kernel: C written in 16-byte groups of four m positions
+    ksplit: int = 1        # packed, one tile per workgroup: k range split into this many parts
     notes: List[str] = field(default_factory=list)
 
     @property
@@ -369,7 +370,10 @@ class GettPlan:
         if not self.packed:
             return 0
         per = _PLANES[self.prec] * _ELEM_BYTES[self.prec] * self.Kp
-        return (-(-(per * self.Mp) // 256) * 256) + per * self.Np
+        ab = (-(-(per * self.Mp) // 256) * 256) + per * self.Np
+        if self.ksplit > 1:
+            return -(-ab // 256) * 256 + self.ksplit * self.Mp * self.Np * (8 if self.prec == F64 else 4)
+        return ab
 
     def describe(self) -> str:
         mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
@@ -385,6 +389,8 @@ class GettPlan:
             walk = " persistent grid=%d" % self.grid
         if self.crun[0] and self.walk != 4:
             walk += " runs=%dx%d" % self.crun
+        if self.ksplit > 1:
+            walk += " ksplit=%d" % self.ksplit
         return ("prec=%s tile=%d (%dx%dx%d)%s%s m=%s n=%s k=%s modeA=%s modeB=%s tiles=%d Mp=%d Np=%d Kp=%d"
                 % (PREC_NAMES[self.prec], self.tile, self.BM, self.BN, self.BK, " packed" if self.packed else "",
                    walk, self.lin_m.inner,
@@ -465,7 +471,7 @@ def _store_run(offCm: np.ndarray, offCn: np.ndarray, BM: int) -> int:
 
 
 def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
-                      persistent: bool = False, store_waves: bool = False) -> GettPlan:
+                      persistent: bool = False, store_waves: bool = False, ksplit: int = 1) -> GettPlan:
     """Plan of the packed path: the reference's pack-then-macro-kernel GETT
     (tccg/gett.py:346-391, tccg/gett.py:1092-1098) with each operand packed
     once.  Plain linearisations (the pack kernel reads every element once, so
@@ -500,6 +506,13 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
         plan.cvec = int(_cvec_ok(offCm, offCn))
         plan.crun = (_store_run(offCm, offCn, BM), 0)
+        return plan
+    if ksplit > 1:
+        # the reference's parallel-K (tccg/gett.py:755-781): k parts into
+        # partial tiles, reduced in a fixed order; one tile per workgroup
+        if persistent or ksplit > plan.Kp // BK:
+            raise ValueError("ksplit %d needs one tile per workgroup and at most %d k-steps" % (ksplit, plan.Kp // BK))
+        plan.ksplit = ksplit
         return plan
     if not persistent:
         esz = 8 if prec == F64 else 4
@@ -652,12 +665,15 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
 # Plan variants as stored by the tuning database: bit 0 packed operands,
 # bit 1 persistent walk, bit 2 streaming walk, bit 3 vector micro-kernel.
 KIND_PACKED, KIND_PERSISTENT, KIND_STREAM, KIND_VECTOR, KIND_STORE_WAVES = 1, 2, 4, 8, 16
+KIND_KSPLIT_SHIFT = 8   # bits 8.. of a plan kind: the ksplit of a split-k plan
 
 
 def plan_kind(plan: GettPlan) -> int:
     if plan.walk == 3:
         return KIND_VECTOR
     kind = KIND_PACKED if plan.packed else 0
+    if plan.ksplit > 1:
+        kind |= plan.ksplit << KIND_KSPLIT_SHIFT
     if plan.walk == 4:
         kind |= KIND_STORE_WAVES
     elif plan.walk == 2:
@@ -690,7 +706,8 @@ def plan_from_kind(pb: GettProblem, prec: int, tile: int, m_blocked: bool, kind:
         return build_vector_plan(pb, prec, tile, groupM=groupM)
     if kind & KIND_PACKED:
         return build_packed_plan(pb, prec, tile, groupM=groupM, persistent=bool(kind & KIND_PERSISTENT),
-                                 store_waves=bool(kind & KIND_STORE_WAVES))
+                                 store_waves=bool(kind & KIND_STORE_WAVES),
+                                 ksplit=max(1, kind >> KIND_KSPLIT_SHIFT))
     return build_plan(pb, prec, tile, m_blocked=m_blocked, groupM=groupM,
                       persistent=bool(kind & KIND_PERSISTENT), stream=bool(kind & KIND_STREAM))
 
@@ -770,7 +787,7 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
                else lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK))
     per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
     slots = hw["cus"] * per_cu
-    ntiles = plan.ntiles()
+    ntiles = plan.ntiles() * plan.ksplit
     eff = 0.75 if plan.BM * plan.BN >= 128 * 128 else 0.55
     t_pack = 0.0
     if plan.packed:
@@ -789,13 +806,13 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
             eff *= 0.85
         # operands split or quantised again for every tile that reads them
         eff *= {BF16X3: 0.45, BF16: 0.6, FP8: 0.6}.get(plan.prec, 1.0)
-    t_mfma = 2.0 * plan.BM * plan.BN * plan.Kp / (hw["peak"][plan.prec] / slots) / eff
+    t_mfma = 2.0 * plan.BM * plan.BN * plan.Kp / plan.ksplit / (hw["peak"][plan.prec] / slots) / eff
     if plan.packed:
         # LDS traffic per tile: every k-step's copy into the ring plus the four
         # waves' fragment reads (each wave reads its half of A's and B's rows),
         # and for the store-wave kernel the staged tile written and read once;
         # 128 bytes per clock per CU
-        ring = 3.0 * _PLANES[plan.prec] * _ELEM_BYTES[plan.prec] * (plan.BM + plan.BN) * plan.Kp
+        ring = 3.0 * _PLANES[plan.prec] * _ELEM_BYTES[plan.prec] * (plan.BM + plan.BN) * plan.Kp / plan.ksplit
         staged = 2.0 * 4 * plan.BM * plan.BN if plan.walk == 4 else 0.0
         t_lds = (ring + staged) / (128 * hw["clock_ghz"] * 1e9) * (hw["cus"] / slots)
         # measured: the compute waves' LDS traffic adds to their MFMA time when
@@ -815,6 +832,9 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
         t = rounds * (max(t_mfma, t_store) + 0.3e-6) + t_pack
     else:
         t = rounds * (t_mfma + t_fixed) + t_pack
+    if plan.ksplit > 1:
+        # the reduction reads ksplit partial tiles and writes C once
+        t += 3e-6 + (plan.ksplit + 1) * esz * plan.Mp * plan.Np / (0.6 * hw["hbm_bytes_per_s"])
     bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
     t_hbm = bytes_ / (0.8 * hw["hbm_bytes_per_s"])
     return max(t, t_hbm), t
@@ -850,6 +870,16 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
                 plans.append(pp)
         if sw_built(prec, tile):
             plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, store_waves=True))
+        # parallel-K when the tiles fill less than two rounds of the GPU and
+        # k is long (the reference splits k when M*N is small next to the
+        # threads, tccg/gett.py:933-936)
+        BM, BN, BK = PACKED_TILES[prec][tile]
+        kt = (-(-pb.K // BK) * BK) // BK
+        nt = (-(-pb.M // BM)) * (-(-pb.N // BN))
+        if nt < 2 * MI355X["cus"] and kt >= 16:
+            for ks in (2, 3, 4):
+                if ks <= kt // 8:
+                    plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, ksplit=ks))
     # ties at the HBM floor are broken by the time the tile work alone would take
     plans.sort(key=lambda p: _estimate(p))
     return plans

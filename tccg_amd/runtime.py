@@ -34,6 +34,7 @@ class _CPlan(ctypes.Structure):
         ("groupM", ctypes.c_int32), ("grid", ctypes.c_int32),
         ("amax", ctypes.c_void_p), ("walk", ctypes.c_int32),
         ("crun_m", ctypes.c_int32), ("crun_n", ctypes.c_int32), ("cvec", ctypes.c_int32),
+        ("ksplit", ctypes.c_int32),
     ]
 
 
@@ -211,6 +212,7 @@ class DevicePlan:
         else:
             c.crun_m, c.crun_n = plan.crun if not plan.grid else (0, 0)
         c.cvec = plan.cvec
+        c.ksplit = plan.ksplit
         c.amax = self.amax.data_ptr() if self.amax is not None else None
         self.cplan = c
         self.workspace = None

@@ -60,12 +60,15 @@ def test_cli_testing_on_gpu(tmp_path):
     assert proc2.returncode == 0 and "[database] reusing" in proc2.stdout
 
 
-@pytest.mark.parametrize("wl,prec,tol", [("abcd-aebf-dfce", 2, 1e-5), ("abcdef-dgfa-gceb", 3, 8e-2)])
-def test_generated_packed_library(wl, prec, tol):
+@pytest.mark.parametrize("wl,prec,suffix,tol", [("abcd-aebf-dfce", 2, "_packed", 1e-5),
+                                                ("abcdef-dgfa-gceb", 3, "_packed", 8e-2),
+                                                ("abcdef-dgfa-gceb", 3, "_storewaves", 8e-2)])
+def test_generated_packed_library(wl, prec, suffix, tol):
     """Generated libraries of packed plans (pack kernels + DMA-ring GETT kernel,
-    workspace allocated by the library on its first call)."""
+    or the store-wave kernel; workspace allocated by the library on its first
+    call)."""
     import torch
-    lib_path = codegen.GENERATED_DIR / ("lib%s_packed.so" % codegen.workload_library_name(wl, prec))
+    lib_path = codegen.GENERATED_DIR / ("lib%s%s.so" % (codegen.workload_library_name(wl, prec), suffix))
     assert lib_path.exists(), "generated library missing: run __graft_entry__.build()"
     g = GeneratedGett(lib_path)
     assert g.info["prec"] == prec

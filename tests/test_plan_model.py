@@ -167,3 +167,64 @@ def test_store_wave_run_order():
     c3 = Contraction.from_tccg(WORKLOADS["abcdef-dgfa-gceb"])
     t = PACKED_TILES[FP8].index((192, 64, 128))
     assert build_packed_plan(c3.problem, FP8, t, store_waves=True).crun[0] == 24
+
+
+def emulate_split(plan, A, B, C, alpha, beta):
+    """Split-k plans: work item (ks, tile) sums k-steps [ks*KT/S, (ks+1)*KT/S)
+    of the tile into a partial tile; the reduction adds the parts in order and
+    writes alpha * sum + beta * C through C's tables."""
+    pb = plan.problem
+    opA, opB = (B, A) if pb.swapped else (A, B)
+    a, b, c = opA.reshape(-1), opB.reshape(-1), C.reshape(-1).copy()
+    t = {key: getattr(plan, key).astype(np.int64) for key in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn")}
+    KT, S = plan.Kp // plan.BK, plan.ksplit
+    tilesM, tilesN = plan.Mp // plan.BM, plan.Np // plan.BN
+    part = np.zeros((S, tilesM * tilesN, plan.BN, plan.BM))
+    covered = np.zeros(KT, dtype=int)
+    for ks in range(S):
+        k0, k1 = ks * KT // S, (ks + 1) * KT // S
+        if ks == 0:
+            assert k0 == 0
+        covered[k0:k1] += 1
+        ks_idx = np.arange(k0 * plan.BK, k1 * plan.BK)
+        ak, bk = t["offAk"][ks_idx], t["offBk"][ks_idx]
+        vk = ak != INV
+        for tm in range(tilesM):
+            am = t["offAm"][tm * plan.BM:(tm + 1) * plan.BM]
+            for tn in range(tilesN):
+                bn = t["offBn"][tn * plan.BN:(tn + 1) * plan.BN]
+                Am = np.where((am[:, None] != INV) & vk[None, :], a[np.where(am[:, None] != INV, am[:, None], 0)
+                                                                   + np.where(vk, ak, 0)[None, :]], 0.0)
+                Bm = np.where((bn[:, None] != INV) & vk[None, :], b[np.where(bn[:, None] != INV, bn[:, None], 0)
+                                                                   + np.where(vk, bk, 0)[None, :]], 0.0)
+                part[ks, tm * tilesN + tn] = Bm @ Am.T          # rows n of BM
+    assert np.all(covered == 1)          # every k-step in exactly one part
+    total = part.sum(axis=0)
+    for tm in range(tilesM):
+        for tn in range(tilesN):
+            cm = t["offCm"][tm * plan.BM:(tm + 1) * plan.BM]
+            cn = t["offCn"][tn * plan.BN:(tn + 1) * plan.BN]
+            for n in range(plan.BN):
+                for m in range(plan.BM):
+                    if cm[m] != INV and cn[n] != INV:
+                        i = cm[m] + cn[n]
+                        c[i] = alpha * total[tm * tilesN + tn, n, m] + beta * c[i]
+    return c.reshape(C.shape)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_split_k_plans(seed):
+    from tccg_amd.plan import BF16X3, PACKED_TILES, build_packed_plan
+    rng = np.random.default_rng(200 + seed)
+    sizes = dict(a=int(rng.integers(3, 40)), b=int(rng.integers(3, 40)), k=int(rng.integers(70, 200)))
+    c = Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"], sizes)
+    sa, sb, sc = c.shapes()
+    A, B, C = rng.uniform(-1, 1, sa), rng.uniform(-1, 1, sb), rng.uniform(-1, 1, sc)
+    ref = contract_ref(["a", "b"], ["a", "k"], ["k", "b"], A, B, C, 0.75, 1.5)
+    for prec in (F32, BF16X3):
+        tile = PACKED_TILES[prec].index((128, 128, 16))
+        for ks in (2, 3, 5):
+            plan = build_packed_plan(c.problem, prec, tile, ksplit=ks)
+            assert plan.ksplit == ks and plan.grid == 0
+            out = emulate_split(plan, A, B, C, 0.75, 1.5)
+            assert rel_frobenius(out, ref) < 1e-13, (sizes, ks)
