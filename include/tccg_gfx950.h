@@ -178,7 +178,9 @@ int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float 
  * modeB only select the read order of the pack kernel: TCCG_KDIM4/KDIM1 read
  * along k, TCCG_FREE4/FREE1 along the free set.  `workspace` is device
  * memory of at least tccg_packed_workspace_bytes() bytes, 16-byte aligned,
- * owned by the caller and not shared by concurrent calls.  TCCG_FP8 reads the
+ * owned by the caller and not shared by concurrent calls; the figure ends
+ * with 160 KiB that the GETT kernel's prefetch may read past the operands
+ * (never writes), so the whole span must be readable device memory.  TCCG_FP8 reads the
  * plan's amax as tccg_gett_fp8 does.
  */
 size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan);

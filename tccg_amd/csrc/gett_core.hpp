@@ -35,6 +35,11 @@
 
 namespace tccg {
 
+// Bytes of readable workspace after a packed plan's operands (and partial
+// tiles): the one-tile walk of the packed kernel prefetches up to one ring
+// (at most the LDS size) past the end of its k range.
+constexpr size_t kPackedTailBytes = 160 * 1024;
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));

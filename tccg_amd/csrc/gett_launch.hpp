@@ -301,9 +301,10 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
 
 
 // Bytes of a packed plan's workspace: A's planes, padded to 256 bytes, B's
-// planes, and with ksplit > 1 (padded to 256 bytes) the ksplit partial C
-// tiles of every tile (FP64 for FP64, else FP32).  0 if the plan's shape is
-// not one of the packed tiles.
+// planes, with ksplit > 1 (padded to 256 bytes) the ksplit partial C tiles
+// of every tile (FP64 for FP64, else FP32), then kPackedTailBytes that the
+// kernel's prefetch may read (never write).  0 if the plan's shape is not
+// one of the packed tiles.
 inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB, size_t* offP = nullptr) {
   int ntiles = 0;
   const TileCfg* tiles = packed_tiles_of(prec, &ntiles);
@@ -323,8 +324,9 @@ inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_
   const size_t ab = (a + b + 255) & ~(size_t)255;
   if (offP) *offP = ab;
   if (plan->ksplit > 1)
-    return ab + (size_t)plan->ksplit * (size_t)plan->Mp * (size_t)plan->Np * (prec == tccg::F64 ? 8 : 4);
-  return a + b;
+    return ab + (size_t)plan->ksplit * (size_t)plan->Mp * (size_t)plan->Np * (prec == tccg::F64 ? 8 : 4) +
+           tccg::kPackedTailBytes;
+  return a + b + tccg::kPackedTailBytes;
 }
 
 // Stages of the LDS ring of the packed GETT kernel.
@@ -530,8 +532,7 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   const size_t need = packed_workspace_bytes(P, plan, &offB, &offP);
   if (workspace_bytes < need) return fail(1, "workspace too small for the packed operands");
   // the packed kernels address elements with 32-bit offsets
-  const size_t endB = plan->ksplit > 1 ? offP : need;
-  if ((endB - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
+  if ((offP - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
     return fail(1, "packed operand exceeds 2^32-1 elements");
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   if (plan->grid < 0) return fail(1, "grid must be >= 0");

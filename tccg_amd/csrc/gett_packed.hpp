@@ -76,6 +76,8 @@ struct PackedShape {
   // copies run S steps ahead with fragment double-buffering (S - 1 without, FP8)
   static_assert(S >= (P == FP8 ? 2 : 3) && S * PER_WAVE < 64, "ring depth");
   static_assert(LDS_BYTES <= 160 * 1024, "LDS capacity");
+  // the one-tile walk's copies run up to S blocks past an operand's end
+  static_assert(S * (BYTES_A > BYTES_B ? BYTES_A : BYTES_B) <= kPackedTailBytes, "workspace tail pad");
 };
 
 // MFMA fragments from the packed LDS layout (the same fragments as
@@ -145,6 +147,44 @@ __device__ inline void packed_copy(const unsigned char* gA, const unsigned char*
     __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
   }
 }
+
+// This wave's copies of the k-steps of one tile: copy j moves kilobyte
+// i = w + 4 j of the stage (as packed_copy does) from the current k-step's
+// block of A (i < NA) or of B.  The two block pointers are wave-uniform
+// (scalar registers, advanced once per k-step); each copy adds this lane's
+// 32-bit offset inside the block, fixed for the whole walk, so a k-step's
+// copies cost an M0 write and the copy each.  When NA is a multiple of 4 the
+// operand of copy j does not depend on the wave.
+template <int P, int BM, int BN, int BK, int S>
+struct PackedCopier {
+  using SH = PackedShape<P, BM, BN, BK, S>;
+  const unsigned char* tA;
+  const unsigned char* tB;
+  uint32_t voff[SH::PER_WAVE];
+
+  __device__ static inline bool is_a(int w, int j) {
+    if constexpr (SH::NA % 4 == 0) return 4 * j < SH::NA;
+    else return w + 4 * j < SH::NA;
+  }
+  __device__ inline void init(const unsigned char* gA, const unsigned char* gB, int w, int lane) {
+    tA = gA;
+    tB = gB;
+#pragma unroll
+    for (int j = 0; j < SH::PER_WAVE; ++j) {
+      const int i = w + 4 * j;
+      voff[j] = (uint32_t)((is_a(w, j) ? i : i - SH::NA) * 1024 + lane * 16);
+    }
+  }
+  __device__ inline void issue(unsigned char* stage, int w) const {
+#pragma unroll
+    for (int j = 0; j < SH::PER_WAVE; ++j)
+      __builtin_amdgcn_global_load_lds((is_a(w, j) ? tA : tB) + voff[j], stage + (w + 4 * j) * 1024, 16, 0, 0);
+  }
+  __device__ inline void advance() {
+    tA += SH::BYTES_A;
+    tB += SH::BYTES_B;
+  }
+};
 
 // Fragments of one k-step for this wave (all k sub-steps of the step).
 template <int P, int BM, int BN, int BK, int WM, int WN>
@@ -289,25 +329,37 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   // k-steps [k0, k0 + KT) of the tile (all of them without a split)
   const int k0 = (int)((long)ks * p.kTiles / ksplit);
   const int KT = (int)((long)(ks + 1) * p.kTiles / ksplit) - k0;
-  // Copy cursor: the next k-step to fetch.  Past the workgroup's last k-step
-  // it stays there (a harmless re-read), so every step issues the same
-  // number of copies and the vmcnt arithmetic holds.
+  // Copy cursor: the next k-step to fetch.  Every step issues the same
+  // number of copies, so the vmcnt arithmetic holds; past the workgroup's
+  // last k-step the persistent walk re-reads that step until its next tile
+  // begins, the one-tile walk reads on (PackedCopier).
   int cv = v, ck = 0;
   const unsigned char* cA = Ab + ((size_t)tm * p.kTiles + k0) * SH::BYTES_A;
   const unsigned char* cB = Bb + ((size_t)tn * p.kTiles + k0) * SH::BYTES_B;
+  PackedCopier<P, BM, BN, BK, S> cp;
+  if constexpr (!PERSIST) cp.init(cA, cB, w, lane);
   auto issue = [&](int slot) {
-    packed_copy<P, BM, BN, BK, S>(cA, cB, smem_raw + slot * SH::STAGE_BYTES, w, lane);
-    if (ck + 1 < KT) {
-      ++ck;
-      cA += SH::BYTES_A;
-      cB += SH::BYTES_B;
-    } else if (PERSIST && cv + (int)gridDim.x < nblocks) {
-      cv += gridDim.x;
-      ck = 0;
-      int ctm, ctn;
-      tile_coords(xcd_chunked(cv, nblocks), p.tilesM, p.tilesN, p.groupM, ctm, ctn);
-      cA = Ab + (size_t)ctm * p.kTiles * SH::BYTES_A;
-      cB = Bb + (size_t)ctn * p.kTiles * SH::BYTES_B;
+    if constexpr (!PERSIST) {
+      // Branch-free, so that the copies stay in the MFMAs' scheduling
+      // region: past the work item's last k-step the copies read on into
+      // the following blocks (at most AHEAD blocks past the operand's end,
+      // inside the workspace's tail pad), and nothing reads those stages.
+      cp.issue(smem_raw + slot * SH::STAGE_BYTES, w);
+      cp.advance();
+    } else {
+      packed_copy<P, BM, BN, BK, S>(cA, cB, smem_raw + slot * SH::STAGE_BYTES, w, lane);
+      if (ck + 1 < KT) {
+        ++ck;
+        cA += SH::BYTES_A;
+        cB += SH::BYTES_B;
+      } else if (cv + (int)gridDim.x < nblocks) {
+        cv += gridDim.x;
+        ck = 0;
+        int ctm, ctn;
+        tile_coords(xcd_chunked(cv, nblocks), p.tilesM, p.tilesN, p.groupM, ctm, ctn);
+        cA = Ab + (size_t)ctm * p.kTiles * SH::BYTES_A;
+        cB = Bb + (size_t)ctn * p.kTiles * SH::BYTES_B;
+      }
     }
   };
 
@@ -363,7 +415,15 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       step_sync<(S - 2) * SH::PER_WAVE>();
     }
     const int nslot = (slot + 1 == S) ? 0 : slot + 1;
-    if constexpr (DB) {
+    // One-tile walk: the fragment reads come first in program order (the
+    // compiler cannot tell that the copies write another stage, so it keeps
+    // them after the reads) and the schedule below spreads both over the
+    // MFMAs.  The persistent walk's copies sit behind its tile-change
+    // branch, outside the MFMAs' scheduling region: they go first.
+    if constexpr (DB && !PERSIST) {
+      nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
+      issue(slot);
+    } else if constexpr (DB) {
       issue(slot);
       nxt.read(smem_raw + nslot * SH::STAGE_BYTES, wm, wn, lane);
     } else {
@@ -372,10 +432,29 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     }
     cur.mma(acc);
     slot = nslot;
-    // Issue order inside the step: the copies, then the next step's LDS
-    // reads one after each of the first MFMAs (their latency hides behind
-    // the rest of the MFMAs), then the remaining MFMAs.
-    if constexpr (DB) {
+    // Issue order inside the step: the next step's LDS reads one after each
+    // of the first MFMAs (their latency hides behind the rest of the MFMAs),
+    // then the copies one after each of the following MFMAs, then the
+    // remaining MFMAs; the matrix core is fed from the step's first
+    // instruction on.
+    if constexpr (DB && !PERSIST) {
+      constexpr int NR = FR::NREAD < FR::NMFMA ? FR::NREAD : FR::NMFMA;
+      constexpr int LEFT = FR::NMFMA - NR;
+      constexpr int NV = SH::PER_WAVE < LEFT ? SH::PER_WAVE : LEFT;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x100, FR::NREAD - NR, 0);
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+      }
+      __builtin_amdgcn_sched_group_barrier(0x020, SH::PER_WAVE - NV, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, LEFT - NV, 0);
+    } else if constexpr (DB) {
       constexpr int NR = FR::NREAD < FR::NMFMA ? FR::NREAD : FR::NMFMA;
       __builtin_amdgcn_sched_group_barrier(0x020, SH::PER_WAVE, 0);
 #pragma unroll

@@ -47,6 +47,10 @@ TILES: Dict[int, List[Tuple[int, int, int]]] = {
 }
 # Tile shapes of the packed path (operands converted once into tile-major
 # planes by the pack kernel); must match csrc/gett_launch.hpp's packed tables.
+# Readable bytes after a packed plan's operands: the one-tile walk of the
+# packed kernel prefetches up to one LDS ring past the end of its k range.
+PACKED_TAIL_BYTES = 160 * 1024
+
 PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
     F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16), (128, 64, 16), (64, 128, 16),
           (192, 64, 16)],
@@ -372,8 +376,8 @@ class GettPlan:
         per = _PLANES[self.prec] * _ELEM_BYTES[self.prec] * self.Kp
         ab = (-(-(per * self.Mp) // 256) * 256) + per * self.Np
         if self.ksplit > 1:
-            return -(-ab // 256) * 256 + self.ksplit * self.Mp * self.Np * (8 if self.prec == F64 else 4)
-        return ab
+            ab = -(-ab // 256) * 256 + self.ksplit * self.Mp * self.Np * (8 if self.prec == F64 else 4)
+        return ab + PACKED_TAIL_BYTES
 
     def describe(self) -> str:
         mode = {FREE4: "FREE4", FREE1: "FREE1", KDIM4: "KDIM4", KDIM1: "KDIM1"}
