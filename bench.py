@@ -248,24 +248,26 @@ def run_form(name, wl, prec, args, log):
 
 def run_batch(args, log):
     """512 FP32 contractions from the 4 batch patterns, every extent uniform over
-    multiples of 8 in [8, 96] (seed 0); tiles tuned per pattern; one
-    HIP-graph-captured persistent launch per step."""
+    multiples of 8 in [8, 96] (seed 0), all in one grouped packed launch (a
+    pack launch and a GETT launch over every problem), its tile and work
+    order chosen by timing the model's best-ranked choices; one
+    HIP-graph-captured step."""
+    import numpy as np
     import torch
-    from tccg_amd.batch import GettBatch, autotune_patterns, random_batch
+    from tccg_amd.batch import GettBatch, autotune_group, random_batch
+    from tccg_amd.plan import F32, PACKED_TILES
     cs = random_batch(512, seed=0)
-    g = torch.Generator(device="cuda").manual_seed(0)
+    rng = np.random.default_rng(0)
     As, Bs, Cs = [], [], []
     for c in cs:
         sa, sb, sc = c.shapes()
-        As.append(torch.rand(sa, device="cuda", generator=g) * 2 - 1)
-        Bs.append(torch.rand(sb, device="cuda", generator=g) * 2 - 1)
+        As.append(torch.from_numpy(rng.uniform(-1, 1, sa).astype(np.float32)).cuda())
+        Bs.append(torch.from_numpy(rng.uniform(-1, 1, sb).astype(np.float32)).cuda())
         Cs.append(torch.zeros(sc, device="cuda"))
     t0 = time.perf_counter()
-    shapes = autotune_patterns(cs, As, Bs, Cs, reps=2, verbose=args.verbose)
+    tile, order, _ = autotune_group(cs, As, Bs, Cs, top=6, reps=2, verbose=args.verbose)
     tune_s = time.perf_counter() - t0
-    # problems of at least 512 packed 128 x 128 tiles (70 of 512, 64 % of the
-    # flops) run on the packed path ahead of the persistent launch
-    b = GettBatch(cs, shapes, packed_min_tiles=512).bind(As, Bs, Cs)
+    b = GettBatch(cs, group_tile=tile, group_order=order).bind(As, Bs, Cs)
     graph = b.capture()
     for _ in range(args.warmup):
         graph.replay()
@@ -285,15 +287,16 @@ def run_batch(args, log):
         ref = fp64_reference(cs[i], As[i], Bs[i])
         errs.append(rel_err(Cs[i], ref))
     flops = b.flops()
+    grp = b._bound["group"]._bound
     res = {"workload": "512 FP32 contractions, 4 patterns, extents U{8..96 step 8}, seed 0",
-           "problems": len(cs), "packed_path_problems": len(b.large), "tiles": b._bound["nwork"],
-           "grid": b._bound["grid"],
-           "pattern_shapes": {k: int(v) for k, v in shapes.items()}, "tune_seconds": round(tune_s, 2),
+           "problems": len(cs), "launches_per_step": b.launches(), "tile": list(PACKED_TILES[F32][tile]),
+           "work_order": order, "work_items": grp["nwork"], "pack_jobs": grp["njobs"],
+           "tune_seconds": round(tune_s, 2),
            "wall_ms_per_step": wall * 1e3, "kernel_ms": kern * 1e3, "gflops": flops / wall / 1e9,
            "kernel_gflops": flops / kern / 1e9, "algorithmic_flops": flops,
            "max_rel_err_vs_fp64_sampled": max(errs)}
-    log("batch   512 problems        %9.1f GFLOP/s (graph replay %.3f ms, %d tiles)" % (
-        res["kernel_gflops"], res["kernel_ms"], res["tiles"]))
+    log("batch   512 problems        %9.1f GFLOP/s (graph replay %.3f ms, %d launches, tile %s, order %s)" % (
+        res["kernel_gflops"], res["kernel_ms"], res["launches_per_step"], res["tile"], order))
     del As, Bs, Cs, b, graph
     torch.cuda.empty_cache()
     return res
@@ -407,31 +410,39 @@ def run_mfma(args, log):
 
 
 def run_log(args, log):
-    """LoG (the reference's loop-over-GEMM method: single-index GEMMs batched
-    over the free loops, contracted loops as accumulating launches) on the
-    compute-bound contraction in FP32, the model's best candidate."""
+    """LoG (the reference's loop-over-GEMM method: every instance of the
+    single-index GEMM one problem of a grouped packed launch, the contracted
+    loops folded into k) on the compute-bound contraction, the model's best
+    candidate, in FP32 and FP64."""
     import torch
     from tccg_amd import Contraction
     from tccg_amd.autotune import time_plan
     from tccg_amd.loop_gemm import DeviceLoG, best_log_candidates, log_candidates
+    from tccg_amd.plan import F32, F64, PACKED_TILES, PREC_NAMES
     from tccg_amd.workloads import get
     c = Contraction.from_tccg(get("abcd-aebf-dfce"))
     cand = best_log_candidates(log_candidates(c.problem), 1)[0]
-    sa, sb, sc = c.shapes()
-    A, B = seeded_inputs(c, torch.float32)
-    C = torch.zeros(sc, dtype=torch.float32, device="cuda")
-    d = DeviceLoG(c, cand, A, B, C, 1.0, 0.0)
-    t = time_plan(d, A, B, C, reps=3)
-    C.zero_()
-    d.launch()
-    torch.cuda.synchronize()
-    err = rel_err(C, fp64_reference(c, A, B))
-    res = {"workload": str(c), "candidate": "loopGemm_" + cand.name, "launches": len(d.launches),
-           "problems_per_launch": len(d.layout.free_bases), "call_ms": t * 1e3, "gflops": c.flops() / t / 1e9,
-           "rel_err_vs_fp64": err}
-    log("log     %-18s %9.1f GFLOP/s (call %.3f ms, %d launches of %d GEMMs) %s"
-        % ("abcd-aebf-dfce", res["gflops"], t * 1e3, res["launches"], res["problems_per_launch"], res["candidate"]))
-    return res
+    out = {}
+    for prec, dt in ((F32, torch.float32), (F64, torch.float64)):
+        A, B = seeded_inputs(c, dt)
+        C = torch.zeros(c.C0.torch_shape(), dtype=dt, device="cuda")
+        d = DeviceLoG(c, cand, A, B, C, 1.0, 0.0, prec=prec)
+        t = time_plan(d, A, B, C, reps=3)
+        C.zero_()
+        d.launch()
+        torch.cuda.synchronize()
+        err = rel_err(C, fp64_reference(c, A, B))
+        lay = d.layout
+        name = PREC_NAMES[prec]
+        out[name] = {"workload": str(c), "candidate": "loopGemm_" + cand.name, "launches": d.nlaunches,
+                     "gemm_instances": len(lay.free_bases), "k_folded": lay.K,
+                     "tile": list(PACKED_TILES[prec][lay.tile]), "call_ms": t * 1e3, "gflops": c.flops() / t / 1e9,
+                     "rel_err_vs_fp64": err}
+        log("log     %-7s %-18s %9.1f GFLOP/s (call %.3f ms, %d launches, %d GEMMs of k %d) %s" % (
+            name, "abcd-aebf-dfce", out[name]["gflops"], t * 1e3, d.nlaunches, len(lay.free_bases), lay.K,
+            out[name]["candidate"]))
+        del A, B, C, d
+    return out
 
 
 def run_grad(args, log):

@@ -266,6 +266,66 @@ int tccg_gett_batch_f32(const tccg_batch_problem* problems, const tccg_batch_wor
                         void* stream);
 
 /*
+ * Grouped packed GETT: many contractions of one precision and one packed
+ * tile shape in TWO launches on `stream`, a pack launch and a GETT launch;
+ * no allocation, memset or host synchronisation, so the pair can be
+ * captured in a HIP graph.  No counterpart exists in the reference: it
+ * serves batches of uneven contractions and the loop-over-GEMM method
+ * (LoG, tccg/gemm_loop.py:76-314), whose GEMM instances become one group.
+ *
+ * Device arrays, all in the caller's memory:
+ *  - `ops`: the operands to pack.  Operand X with tables offF (Fp entries,
+ *    Fp = 64 * fBlocks, a multiple of the tile's BM, or BN when isB) and
+ *    offK (kTiles * BK entries, a multiple of 32) is converted like
+ *    tccg_gett_packed_<prec> packs an operand, into the workspace at byte
+ *    offset `out`.  Several problems may share one packed operand (LoG: the
+ *    instances that differ only in an index the operand lacks).
+ *  - `jobs`: one entry (item, f, k) per 64 x 32 block of every operand,
+ *    f < fBlocks, k < kTiles * BK / 32.
+ *  - `problems`: per contraction the byte offsets of its packed A (rows of
+ *    BM) and B (rows of BN) planes, C with its tables offCm (tilesM * BM
+ *    entries) and offCn (tilesN * BN), the k-steps kTiles, alpha and beta.
+ *  - `work`: the C tiles (problem, tm, tn), one workgroup each in list order
+ *    (workgroup i runs on XCD i mod 8): every tile of every problem exactly
+ *    once; entries with problem < 0 are padding and skipped.
+ * The workspace must hold every packed plane and, after the last one,
+ * kPackedTailBytes (160 KiB) of readable memory (see
+ * tccg_packed_workspace_bytes).  Entries whose indices or planes fall
+ * outside the arrays or the workspace are skipped; the tables' contents are
+ * the caller's to get right, as for tccg_gett_packed_<prec>.
+ * Precisions: TCCG_F32, TCCG_F64, TCCG_BF16X3 and TCCG_BF16 (FP64 data for
+ * TCCG_F64, FP32 otherwise).  Returns 0 or an error code (tccg_last_error).
+ */
+typedef struct tccg_group_operand {
+  const void* X;
+  const uint32_t* offF;
+  const uint32_t* offK;
+  uint64_t out;
+  int32_t fBlocks;
+  int32_t kTiles;
+  int32_t kfast; /* 1: gather along k (TCCG_KDIM4/KDIM1), 0: along the free set */
+  int32_t isB;   /* 0: rows of BM (an A operand), 1: rows of BN */
+} tccg_group_operand;
+
+typedef struct tccg_group_job {
+  int32_t item, f, k, pad;
+} tccg_group_job;
+
+typedef struct tccg_group_problem {
+  void* C;
+  const uint32_t* offCm;
+  const uint32_t* offCn;
+  uint64_t packedA;
+  uint64_t packedB;
+  int32_t tilesM, tilesN, kTiles, pad;
+  double alpha, beta;
+} tccg_group_problem;
+
+int tccg_gett_group(int prec, int tile, const tccg_group_operand* ops, int nops, const tccg_group_job* jobs,
+                    int njobs, const tccg_group_problem* problems, int nproblems, const tccg_batch_work* work,
+                    int nwork, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Tensor transposition out = alpha * permute(in) + beta * out, the step the
  * reference's TTGT method runs before and after its GEMM (Transpose,
  * tccg/transpose.py:74-144, executed there through HPTT's

@@ -54,13 +54,23 @@ class GettBatch:
 
     def __init__(self, contractions: Sequence[Contraction], shapes: Optional[Dict[str, int]] = None,
                  alphas: Optional[Sequence[float]] = None, betas: Optional[Sequence[float]] = None,
-                 packed_min_tiles: Optional[int] = None):
+                 packed_min_tiles: Optional[int] = None, group_tile: Optional[int] = None,
+                 group_order: str = "xcd"):
         self.contractions = list(contractions)
         self.shapes = dict(shapes or {})
         self.alphas = list(alphas) if alphas is not None else [c.alpha for c in self.contractions]
         self.betas = list(betas) if betas is not None else [c.beta for c in self.contractions]
         self.plans = []
         self.large: Dict[int, object] = {}      # problem index -> packed plan
+        # group_tile: every problem on the grouped packed path (one pack
+        # launch and one GETT launch, tccg_amd.group) with this FP32 packed
+        # tile and work order
+        self.group_tile = group_tile
+        self.group_order = group_order
+        if group_tile is not None:
+            self.plans = [build_packed_plan(c.problem, F32, group_tile) for c in self.contractions]
+            self._bound = None
+            return
         for i, c in enumerate(self.contractions):
             if packed_min_tiles is not None:
                 pp = build_packed_plan(c.problem, F32, LARGE_TILE)
@@ -86,6 +96,12 @@ class GettBatch:
         lib = load_library()
         from .runtime import cuda_device
         dev = cuda_device(torch, device if device is not None else As[0].device)
+        self._check_tensors(As, Bs, Cs, dev)
+        if self.group_tile is not None:
+            from .group import group_for_plans
+            grp = group_for_plans(self.plans, As, Bs, Cs, self.alphas, self.betas, self.group_order).bind(dev)
+            self._bound = dict(group=grp, tensors=(list(As), list(Bs), list(Cs)))
+            return self
         # problems on the packed path: one shared workspace (they run one after the other)
         large = []
         if self.large:
@@ -115,10 +131,6 @@ class GettBatch:
         for j in range(len(small)):
             i = small[j]
             p, c, A, B, C = self.plans[i], self.contractions[i], As[i], Bs[i], Cs[i]
-            for name, t, T in (("A", A, c.A0), ("B", B, c.B0), ("C", C, c.C0)):
-                if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device != dev:
-                    raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements on %s"
-                                     % (i, name, T.size(), dev))
             opA, opB = (B, A) if p.problem.swapped else (A, B)
             q = probs[j]
             q.A, q.B, q.C = opA.data_ptr(), opB.data_ptr(), C.data_ptr()
@@ -144,12 +156,34 @@ class GettBatch:
                            tensors=(list(As), list(Bs), list(Cs)), lib=lib, large=large)
         return self
 
+    def _check_tensors(self, As, Bs, Cs, dev) -> None:
+        import torch
+        if not len(As) == len(Bs) == len(Cs) == len(self.contractions):
+            raise ValueError("one A, B and C tensor per contraction")
+        for i, c in enumerate(self.contractions):
+            for name, t, T in (("A", As[i], c.A0), ("B", Bs[i], c.B0), ("C", Cs[i], c.C0)):
+                if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device != dev:
+                    raise ValueError("problem %d: %s must be a contiguous float32 tensor of %d elements on %s"
+                                     % (i, name, T.size(), dev))
+
+    def launches(self) -> int:
+        """Kernel launches of one batch step."""
+        b = self._bound
+        if b is None:
+            raise RuntimeError("bind() the batch to tensors first")
+        if "group" in b:
+            return 2
+        return 3 * len(b["large"]) + (1 if b["nwork"] else 0)
+
     def launch(self, stream=None) -> None:
         import torch
         from .runtime import GettRuntimeError
         b = self._bound
         if b is None:
             raise RuntimeError("bind() the batch to tensors first")
+        if "group" in b:
+            b["group"].launch(stream)
+            return
         dev = b["work"].device
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         As, Bs, Cs = b["tensors"]
@@ -224,4 +258,48 @@ def autotune_patterns(contractions: Sequence[Contraction], As, Bs, Cs, reps: int
             import sys
             print("  pattern %-14s %3d problems: %s -> shape %d" % (
                 key, len(idx), {k: "%.1f us" % (v * 1e6) for k, v in times.items()}, best[key]), file=sys.stderr)
+    return best
+
+
+def group_candidates(contractions: Sequence[Contraction], betas: Optional[Sequence[float]] = None):
+    """(tile, order) choices of the grouped launch ranked by the planner's
+    model of a grouped launch (plan.group_estimate: every tile of every
+    problem sharing the GPU; the gfx950 analogue of the reference's
+    model-ranked candidates, tccg/gett.py:987-1150), each tile with both
+    work orders."""
+    from .plan import group_estimate
+    betas = list(betas) if betas is not None else [c.beta for c in contractions]
+    est = []
+    for t in range(len(PACKED_TILES[F32])):
+        est.append((group_estimate([build_packed_plan(c.problem, F32, t) for c in contractions], betas), t))
+    est.sort()
+    return [(t, order, e) for e, t in est for order in ("xcd", "lpt")]
+
+
+def autotune_group(contractions: Sequence[Contraction], As, Bs, Cs, top: int = 6, reps: int = 2,
+                   verbose: bool = False):
+    """Time the `top` model-ranked (tile, work order) choices of the grouped
+    launch on the bound tensors (the reference's measure-and-select,
+    tccg/tccg.py:278-430) and return the fastest as (tile, order, seconds)."""
+    import torch
+    best = None
+    for tile, order, est in group_candidates(contractions)[:top]:
+        b = GettBatch(contractions, group_tile=tile, group_order=order).bind(As, Bs, Cs)
+        b.launch()
+        torch.cuda.synchronize()
+        t = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            b.launch()
+            e1.record()
+            e1.synchronize()
+            t = min(t, e0.elapsed_time(e1) * 1e-3)
+        if verbose:
+            import sys
+            print("  group tile %s order %s: model %.2f ms, measured %.2f ms" % (
+                PACKED_TILES[F32][tile], order, est * 1e3, t * 1e3), file=sys.stderr)
+        if best is None or t < best[2]:
+            best = (tile, order, t)
+        del b
     return best

@@ -34,9 +34,10 @@ HIP_FLAGS = [
 
 LIB_UNITS = ["gett_common.hip", "gett_f32.hip", "gett_f64.hip", "gett_bf16x3.hip", "gett_fp8.hip", "gett_bf16.hip",
              "gett_batch.hip", "gett_packed_f32.hip", "gett_packed_f64.hip", "gett_packed_bf16x3.hip",
-             "gett_packed_bf16.hip", "gett_packed_fp8.hip", "transpose.hip", "gett_noalloc.hip"]
+             "gett_packed_bf16.hip", "gett_packed_fp8.hip", "transpose.hip", "gett_noalloc.hip",
+             "gett_group_f32.hip", "gett_group_f64.hip", "gett_group_bf16x3.hip", "gett_group_bf16.hip"]
 LIB_HEADERS = ["gett_core.hpp", "gett_launch.hpp", "gett_pack.hpp", "gett_packed.hpp", "gett_packed_sw.hpp",
-               "gett_vector.hpp", "transpose.hpp"]
+               "gett_vector.hpp", "transpose.hpp", "gett_group.hpp"]
 
 
 class CompilationError(RuntimeError):

@@ -279,15 +279,16 @@ def _check_result(name: str, prec: int, C, ref, tol: float) -> bool:
 
 def _time_log(args, contraction, prec):
     """Time the LoG candidates with the largest GEMM and batch (at most
-    --maxImplementationsLoG; FP32 only: the batched launch computes in FP32).
-    Best GFLOP/s (0 when LoG does not apply), or None if --testing found a
-    wrong result."""
+    --maxImplementationsLoG; the grouped launch runs FP32, FP64, split-BF16
+    and BF16).  Best GFLOP/s (0 when LoG does not apply), or None if
+    --testing found a wrong result."""
     import torch
     from .autotune import time_plan
+    from .group import GROUP_PRECS
     from .loop_gemm import DeviceLoG, best_log_candidates, log_candidates
     cands = log_candidates(contraction.problem)
-    if cands is None or prec != F32:
-        print("LoG not possible" if cands is None else "LoG: FP32 only (skipped for %s)" % PREC_NAMES[prec])
+    if cands is None or prec not in GROUP_PRECS:
+        print("LoG not possible" if cands is None else "LoG: not run in %s" % PREC_NAMES[prec])
         return 0.0
     best = best_log_candidates(cands, args.maxImplementationsLoG)
     print("Total amount of LoG implementations: %d" % len(cands))
@@ -296,7 +297,7 @@ def _time_log(args, contraction, prec):
     C = C0.clone()
     top, top_name = -1.0, ""
     for cand in best:
-        d = DeviceLoG(contraction, cand, A, B, C)
+        d = DeviceLoG(contraction, cand, A, B, C, prec=prec)
         name = "loopGemm_" + cand.name
         if ref is not None:
             C.copy_(C0)

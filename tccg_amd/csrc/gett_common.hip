@@ -90,6 +90,39 @@ int tccg_vector_tile_shape(int prec, int tile, int* BM, int* BN, int* BK) {
   return 0;
 }
 
+int tccg_gett_group_f32(int, const tccg_group_operand*, int, const tccg_group_job*, int, const tccg_group_problem*,
+                        int, const tccg_batch_work*, int, void*, size_t, void*);
+int tccg_gett_group_f64(int, const tccg_group_operand*, int, const tccg_group_job*, int, const tccg_group_problem*,
+                        int, const tccg_batch_work*, int, void*, size_t, void*);
+int tccg_gett_group_bf16x3(int, const tccg_group_operand*, int, const tccg_group_job*, int,
+                           const tccg_group_problem*, int, const tccg_batch_work*, int, void*, size_t, void*);
+int tccg_gett_group_bf16(int, const tccg_group_operand*, int, const tccg_group_job*, int, const tccg_group_problem*,
+                         int, const tccg_batch_work*, int, void*, size_t, void*);
+
+static_assert(sizeof(tccg_group_operand) == 48 && sizeof(tccg_group_problem) == 72 && sizeof(tccg_group_job) == 16,
+              "group record layouts are part of the ABI");
+
+int tccg_gett_group(int prec, int tile, const tccg_group_operand* ops, int nops, const tccg_group_job* jobs,
+                    int njobs, const tccg_group_problem* problems, int nproblems, const tccg_batch_work* work,
+                    int nwork, void* workspace, size_t workspace_bytes, void* stream) {
+  switch (prec) {
+    case tccg::F32:
+      return tccg_gett_group_f32(tile, ops, nops, jobs, njobs, problems, nproblems, work, nwork, workspace,
+                                 workspace_bytes, stream);
+    case tccg::F64:
+      return tccg_gett_group_f64(tile, ops, nops, jobs, njobs, problems, nproblems, work, nwork, workspace,
+                                 workspace_bytes, stream);
+    case tccg::BF16X3:
+      return tccg_gett_group_bf16x3(tile, ops, nops, jobs, njobs, problems, nproblems, work, nwork, workspace,
+                                    workspace_bytes, stream);
+    case tccg::BF16:
+      return tccg_gett_group_bf16(tile, ops, nops, jobs, njobs, problems, nproblems, work, nwork, workspace,
+                                  workspace_bytes, stream);
+    default:
+      return tccg_lib::fail(1, "the grouped GETT runs FP32, FP64, split-BF16 and BF16");
+  }
+}
+
 int tccg_packed_sw_available(int prec, int tile) {
   using namespace tccg_lib;
   switch (prec) {

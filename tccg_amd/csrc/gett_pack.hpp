@@ -102,25 +102,22 @@ constexpr int kPackTK = 32;  // k positions per pack workgroup
 // operand: gathered into LDS with the lanes along the operand's stride-one
 // index (kfast != 0: the k set, else the free set), then written with each
 // thread converting eight consecutive k values of one row, consecutive
-// threads on consecutive output chunks.  Grid: (Fp / 64, Kp / 32).
-// amax (FP8 only) points at max|X|; the scale maps it onto 448.
+// threads on consecutive output chunks.  amax (FP8 only) points at max|X|;
+// the scale maps it onto 448.
+// The 64 x 32 block at free position f0, k position k0 (tile: the
+// workgroup's LDS staging buffer).
 template <int P, int F, int BK>
-__global__ void __launch_bounds__(256)
-pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __restrict__ offF,
-            const uint32_t* __restrict__ offK, int kTiles, int kfast, const float* __restrict__ amax,
-            typename PrecTraits<P>::TL* __restrict__ out) {
+__device__ __forceinline__ void pack_block(const typename PrecTraits<P>::TG* __restrict__ X,
+                                           const uint32_t* __restrict__ offF, const uint32_t* __restrict__ offK,
+                                           int kTiles, int kfast, const float* __restrict__ amax,
+                                           typename PrecTraits<P>::TL* __restrict__ out, int f0, int k0,
+                                           typename PrecTraits<P>::TG (*tile)[kPackTK + 1]) {
   using TR = PrecTraits<P>;
   using TG = typename TR::TG;
   constexpr int TF = kPackTF, TK = kPackTK;
   constexpr int BLOCK = TR::PLANES * F * BK;
   static_assert(F % TF == 0, "tile rows are whole pack blocks");
   static_assert(BK % 8 == 0 && (BK <= TK ? TK % BK == 0 : BK % TK == 0), "pack block vs k-step");
-  __shared__ TG tile[TF][TK + 1];
-
-  // consecutive blocks along the free set run on one XCD: neighbours that
-  // share the partial cache lines of a stride-one run meet in the same L2
-  const int lb = xcd_chunked(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int f0 = (lb % gridDim.x) * TF, k0 = (lb / gridDim.x) * TK;
   const int t = threadIdx.x;
   if (kfast) {
     const int k = t & (TK - 1);
@@ -166,6 +163,20 @@ pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __
   const int gf = f0 + f, gk = k0 + kl;
   const size_t blk = (size_t)(gf / F) * (size_t)kTiles * BLOCK + (size_t)(gk / BK) * BLOCK;
   PackWriter<P>::write8(out + blk, F, F * BK, gf % F, gk % BK, v, scale);
+}
+
+// Grid: (Fp / 64, Kp / 32), one block per workgroup.
+template <int P, int F, int BK>
+__global__ void __launch_bounds__(256)
+pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __restrict__ offF,
+            const uint32_t* __restrict__ offK, int kTiles, int kfast, const float* __restrict__ amax,
+            typename PrecTraits<P>::TL* __restrict__ out) {
+  __shared__ typename PrecTraits<P>::TG tile[kPackTF][kPackTK + 1];
+  // consecutive blocks along the free set run on one XCD: neighbours that
+  // share the partial cache lines of a stride-one run meet in the same L2
+  const int lb = xcd_chunked(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int f0 = (lb % gridDim.x) * kPackTF, k0 = (lb / gridDim.x) * kPackTK;
+  pack_block<P, F, BK>(X, offF, offK, kTiles, kfast, amax, out, f0, k0, tile);
 }
 
 }  // namespace tccg

@@ -298,37 +298,25 @@ __device__ inline void store_tile_staged(const GettParams<typename PrecTraits<P>
 // blockIdx.x walks tiles blockIdx.x, + gridDim.x, ...; the copies and the
 // fragment reads run on across tiles, so the next tile's first k-steps are
 // in flight during the epilogue.
+// The walk of one workgroup: tile (tm, tn), k-steps [k0, k0 + KT) (all of
+// them without a split; part ks of a split-k plan), and with PERSIST the
+// tiles of work items v + gridDim.x, ... < nblocks after it.  Every copy
+// into LDS has landed when it returns.
 template <int P, int BM, int BN, int BK, int WM, int WN, int S, bool PERSIST>
-__global__ void __launch_bounds__(256)
-gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
+__device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits<P>::TG>& p, int v, int nblocks,
+                                            int tm, int tn, int ks, int k0, int KT, unsigned char* smem_raw) {
   using SH = PackedShape<P, BM, BN, BK, S>;
   using TS = TileShape<P, BM, BN, BK, WM, WN>;
   using MM = Mma<P>;
   using FR = PackedFrags<P, BM, BN, BK, WM, WN>;
-  extern __shared__ __align__(1024) unsigned char smem_raw[];
-
   const int ntiles = p.tilesM * p.tilesN;
   const int ksplit = PERSIST ? 1 : p.ksplit;
-  const int nblocks = ntiles * ksplit;   // work items: tiles, or (k part, tile) pairs
-  int v = blockIdx.x;
-  if (v >= nblocks) return;
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wm = w % WM, wn = w / WM;
   const unsigned char* Ab = reinterpret_cast<const unsigned char*>(p.A);
   const unsigned char* Bb = reinterpret_cast<const unsigned char*>(p.B);
-
-  int tm, tn;
-  int lv = xcd_chunked(v, nblocks), ks = 0;
-  if (ksplit > 1) {
-    ks = lv / ntiles;   // consecutive work items share a k part: neighbouring tiles share its blocks
-    lv -= ks * ntiles;
-  }
-  tile_coords(lv, p.tilesM, p.tilesN, p.groupM, tm, tn);
-  // k-steps [k0, k0 + KT) of the tile (all of them without a split)
-  const int k0 = (int)((long)ks * p.kTiles / ksplit);
-  const int KT = (int)((long)(ks + 1) * p.kTiles / ksplit) - k0;
   // Copy cursor: the next k-step to fetch.  Every step issues the same
   // number of copies, so the vmcnt arithmetic holds; past the workgroup's
   // last k-step the persistent walk re-reads that step until its next tile
@@ -511,8 +499,31 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
       for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
     fresh = S - 1;
   }
-  // No copy may still be writing this workgroup's LDS when it retires.
+  // No copy may still be writing this workgroup's LDS when it retires (or
+  // when it starts its next work item).
   wait_vm_lgkm0<0>();
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN, int S, bool PERSIST>
+__global__ void __launch_bounds__(256)
+gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
+  extern __shared__ __align__(1024) unsigned char smem_raw[];
+  const int ntiles = p.tilesM * p.tilesN;
+  const int ksplit = PERSIST ? 1 : p.ksplit;
+  const int nblocks = ntiles * ksplit;   // work items: tiles, or (k part, tile) pairs
+  const int v = blockIdx.x;
+  if (v >= nblocks) return;
+  int tm, tn;
+  int lv = xcd_chunked(v, nblocks), ks = 0;
+  if (ksplit > 1) {
+    ks = lv / ntiles;   // consecutive work items share a k part: neighbouring tiles share its blocks
+    lv -= ks * ntiles;
+  }
+  tile_coords(lv, p.tilesM, p.tilesN, p.groupM, tm, tn);
+  // k-steps [k0, k0 + KT) of the tile (all of them without a split)
+  const int k0 = (int)((long)ks * p.kTiles / ksplit);
+  const int KT = (int)((long)(ks + 1) * p.kTiles / ksplit) - k0;
+  packed_walk<P, BM, BN, BK, WM, WN, S, PERSIST>(p, v, nblocks, tm, tn, ks, k0, KT, smem_raw);
 }
 
 // C = alpha * (sum of the ksplit partial tiles) + beta * C, the partial sums

@@ -13,11 +13,16 @@ Candidates are named as the reference names them (getCandidateName,
 tccg/gemm_loop.py:61-76) and the ones with the largest GEMM, then the largest
 batch, are kept (selectBestCandidate, tccg/gemm_loop.py:313-348).
 
-On the GPU every instance of the GEMM for one combination of the contracted
-loop indices is one problem of a single batched launch (tccg_gett_batch_f32,
-the persistent batched GETT): all problems share one set of offset tables
-over (m, n, k), taken with the full tensors' strides, and differ only in
-their base pointers.  The contracted loops are consecutive launches.
+On the GPU the instances of the GEMM are the problems of one grouped packed
+launch (tccg_gett_group, tccg_amd.group): one problem per combination of the
+free loops, all sharing one set of offset tables over (m, n, k) taken with
+the full tensors' strides and differing only in their base pointers.  The
+contracted loops are folded into the k range of every problem: its k table
+runs over (contracted loop combination, k), so each C tile accumulates the
+whole sum in registers and is written once (alpha and beta applied then),
+where the reference accumulates into C from one GEMM call to the next.
+Operands are packed once per distinct base: the instances that differ only
+in an index an operand lacks share its packed planes.
 """
 from __future__ import annotations
 
@@ -28,7 +33,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .plan import FREE1, FREE4, INVALID, KDIM1, KDIM4, _vector_ok_free, _vector_ok_k
+from .plan import BF16, BF16X3, F32, F64, INVALID, PACKED_TILES
 
 DUMMY = "dummy"
 
@@ -98,65 +103,61 @@ def best_log_candidates(cands: Sequence[LogCandidate], max_candidates: int) -> L
     return [c for c in cands if c.gemm_size() == g and c.sizes[0] == b][:max(1, max_candidates)]
 
 
-def _prod(xs) -> int:
-    p = 1
-    for x in xs:
-        p *= int(x)
-    return p
+def _roundup(x: int, m: int) -> int:
+    return -(-x // m) * m
 
 
-def _table(extent: int, stride: int, pad_to: int) -> np.ndarray:
-    total = -(-extent // pad_to) * pad_to
+def _lcm(a: int, b: int) -> int:
+    import math
+    return a * b // math.gcd(a, b)
+
+
+def _table(values: np.ndarray, total: int) -> np.ndarray:
     out = np.full(total, INVALID, dtype=np.uint32)
-    out[:extent] = (np.arange(extent, dtype=np.int64) * stride).astype(np.uint32)
+    if np.any(values < 0) or np.any(values >= INVALID):
+        raise ValueError("LoG element offsets beyond 32 bits")
+    out[:values.size] = values.astype(np.uint32)
     return out
+
+
+def log_tile(prec: int, Mg: int, Ng: int) -> int:
+    """Packed tile with the least padded m x n area for the GEMM (the
+    larger tile among equals)."""
+    best, key = 0, None
+    for t, (BM, BN, _) in enumerate(PACKED_TILES[prec]):
+        area = _roundup(Mg, _lcm(BM, 64)) * _roundup(Ng, _lcm(BN, 64))
+        k = (area, -BM * BN)
+        if key is None or k < key:
+            best, key = t, k
+    return best
 
 
 @dataclass
 class LogLayout:
-    """Everything the batched launches of one LoG candidate need, host side."""
-    shape: int                                # batch tile shape id (tccg_amd.batch.BATCH_SHAPES)
-    BM: int
-    BN: int
-    BK: int
-    tables: Tuple[np.ndarray, ...]            # offAm, offAk, offBn, offBk, offCm, offCn, ordA, ordB
-    modeA: int
-    modeB: int
-    free_bases: List[Tuple[int, int, int]]    # element offsets into (opA, opB, C), one per problem
-    contracted_bases: List[Tuple[int, int, int]]   # one per launch
+    """The grouped launch of one LoG candidate, host side: GEMM tables shared
+    by every instance (k tables over (contracted combination, k)), the pack
+    read orders, and the element offsets (opA, opB, C) of every instance."""
+    prec: int
+    tile: int
+    tables: Tuple[np.ndarray, ...]            # offAm, offAk, offBn, offBk, offCm, offCn
+    kfastA: int
+    kfastB: int
+    free_bases: List[Tuple[int, int, int]]
 
     @property
-    def tiles(self) -> Tuple[int, int, int]:
-        return self.tables[0].size // self.BM, self.tables[2].size // self.BN, self.tables[1].size // self.BK
+    def K(self) -> int:
+        return int(np.count_nonzero(self.tables[1] != INVALID))
 
 
-def log_layout(pb, cand: LogCandidate, shape: Optional[int] = None) -> LogLayout:
-    """Shared GEMM tables over (m, n, k) with the full tensors' strides, load
-    modes, and the base offsets of every free and contracted loop combination."""
-    from .batch import BATCH_SHAPES
-    from .plan import F32, TILES
+def log_layout(pb, cand: LogCandidate, prec: int = F32, tile: Optional[int] = None) -> LogLayout:
     AA, BB, CC = pb.opA, pb.opB, pb.C
     has = lambda T, l: l in T.labels()
     stride = lambda T, l: T.stride_of_label(l) if (l != DUMMY and has(T, l)) else 0
     ext = lambda l: 1 if l == DUMMY else next(i.size for X in (AA, BB, CC) for i in X.indices if i.label == l)
     Mg, Ng, Kg = ext(cand.m), ext(cand.n), ext(cand.k)
-    if shape is None:
-        shape = 1 if Mg * Ng >= 256 * 256 else 0
-    BM, BN, BK = TILES[F32][BATCH_SHAPES[shape]]
-    offAm, offCm = _table(Mg, stride(AA, cand.m), BM), _table(Mg, stride(CC, cand.m), BM)
-    offBn, offCn = _table(Ng, stride(BB, cand.n), BN), _table(Ng, stride(CC, cand.n), BN)
-    offAk, offBk = _table(Kg, stride(AA, cand.k), BK), _table(Kg, stride(BB, cand.k), BK)
-    ordA, ordB = np.arange(BM, dtype=np.int32), np.arange(BN, dtype=np.int32)
-    a0, b0 = AA.labels()[0], BB.labels()[0]
-    # the stride-one index of A is m or k, that of B n or k (log_candidates)
-    if a0 == cand.m:
-        modeA = FREE4 if _vector_ok_free(offAm, ordA, BM) else FREE1
-    else:
-        modeA = KDIM4 if _vector_ok_k(offAk) else KDIM1
-    if b0 == cand.n:
-        modeB = FREE4 if _vector_ok_free(offBn, ordB, BN) else FREE1
-    else:
-        modeB = KDIM4 if _vector_ok_k(offBk) else KDIM1
+    if tile is None:
+        tile = log_tile(prec, Mg, Ng)
+    BM, BN, BK = PACKED_TILES[prec][tile]
 
     def bases(labels):
         combos = [(0, 0, 0)]
@@ -165,109 +166,68 @@ def log_layout(pb, cand: LogCandidate, shape: Optional[int] = None) -> LogLayout
             combos = [(x + i * sa, y + i * sb, z + i * sc) for i in range(ext(l)) for (x, y, z) in combos]
         return combos
 
+    cb = np.array(bases(list(cand.contracted_order)), dtype=np.int64).reshape(-1, 3)
+    kk = np.arange(Kg, dtype=np.int64)
+    Kp = _roundup(cb.shape[0] * Kg, _lcm(BK, 32))
+    Mp, Np = _roundup(Mg, _lcm(BM, 64)), _roundup(Ng, _lcm(BN, 64))
+    mm, nn = np.arange(Mg, dtype=np.int64), np.arange(Ng, dtype=np.int64)
+    offAk = _table((cb[:, 0:1] + kk[None, :] * stride(AA, cand.k)).reshape(-1), Kp)
+    offBk = _table((cb[:, 1:2] + kk[None, :] * stride(BB, cand.k)).reshape(-1), Kp)
+    offAm, offCm = _table(mm * stride(AA, cand.m), Mp), _table(mm * stride(CC, cand.m), Mp)
+    offBn, offCn = _table(nn * stride(BB, cand.n), Np), _table(nn * stride(CC, cand.n), Np)
+    # gather along each operand's stride-one index (the pack kernel's read order)
+    kfastA = 1 if (cand.k != DUMMY and AA.labels()[0] == cand.k) else 0
+    kfastB = 1 if (cand.k != DUMMY and BB.labels()[0] == cand.k) else 0
     free = ([cand.batch] if cand.batch != DUMMY else []) + list(cand.free_order)
-    return LogLayout(shape, BM, BN, BK, (offAm, offAk, offBn, offBk, offCm, offCn, ordA, ordB), modeA, modeB,
-                     bases(free), bases(list(cand.contracted_order)))
+    return LogLayout(prec, tile, (offAm, offAk, offBn, offBk, offCm, offCn), kfastA, kfastB, bases(free))
 
 
-# tccg_batch_problem (include/tccg_gfx950.h) as a numpy record
-PROBLEM_DTYPE = np.dtype([(n, "<u8") for n in ("A", "B", "C", "offAm", "offAk", "offBn", "offBk", "offCm", "offCn",
-                                               "ordA", "ordB")]
-                         + [(n, "<i4") for n in ("tilesM", "tilesN", "kTiles", "variant")]
-                         + [("alpha", "<f4"), ("beta", "<f4")])
-
-
-def problem_records(lay: LogLayout, ptrA: int, ptrB: int, ptrC: int, tables: Sequence[int], alpha: float,
-                    beta: float) -> List[np.ndarray]:
-    """One record array per launch (combination of the contracted loops):
-    problem i of launch j multiplies opA + free_bases[i] + contracted_bases[j]
-    by opB + ... into C + free_bases[i], FP32 element offsets; beta on the
-    first launch, 1 on the others."""
-    tilesM, tilesN, kTiles = lay.tiles
-    fb = np.array(lay.free_bases, dtype=np.uint64).reshape(-1, 3)
-    rec = np.zeros(fb.shape[0], dtype=PROBLEM_DTYPE)
-    for n, v in zip(("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB"), tables):
-        rec[n] = v
-    rec["tilesM"], rec["tilesN"], rec["kTiles"] = tilesM, tilesN, kTiles
-    rec["variant"] = lay.shape * 16 + lay.modeA * 4 + lay.modeB
-    rec["alpha"] = alpha
-    rec["C"] = np.uint64(ptrC) + np.uint64(4) * fb[:, 2]
-    out = []
-    for j, (ca, cb, _) in enumerate(lay.contracted_bases):
-        rec["A"] = np.uint64(ptrA) + np.uint64(4) * (fb[:, 0] + np.uint64(ca))
-        rec["B"] = np.uint64(ptrB) + np.uint64(4) * (fb[:, 1] + np.uint64(cb))
-        rec["beta"] = beta if j == 0 else 1.0
-        out.append(rec.copy())
-    return out
+_TORCH_DTYPE = {F32: "float32", F64: "float64", BF16X3: "float32", BF16: "float32"}
 
 
 class DeviceLoG:
-    """One LoG candidate bound to fixed FP32 tensors: the shared GEMM tables,
-    one problem table per combination of the contracted loops (the first with
-    beta, the others accumulating), and the work list of the batched launches."""
+    """One LoG candidate bound to fixed tensors: every GEMM instance a problem
+    of one grouped packed launch (a pack launch and a GETT launch)."""
 
     def __init__(self, contraction, cand: LogCandidate, A, B, C, alpha: Optional[float] = None,
-                 beta: Optional[float] = None, shape: Optional[int] = None):
+                 beta: Optional[float] = None, prec: int = F32, tile: Optional[int] = None, order: str = "xcd"):
         import torch
-        from .batch import _CProblem
-        from .runtime import load_library
+        from .group import GettGroup
+        if prec not in _TORCH_DTYPE:
+            raise ValueError("LoG runs FP32, FP64, split-BF16 and BF16")
         pb = contraction.problem
-        self.contraction, self.cand = contraction, cand
-        alpha = contraction.alpha if alpha is None else alpha
-        beta = contraction.beta if beta is None else beta
+        self.contraction, self.cand, self.prec = contraction, cand, prec
+        self.alpha = contraction.alpha if alpha is None else alpha
+        self.beta = contraction.beta if beta is None else beta
+        dt = getattr(torch, _TORCH_DTYPE[prec])
         for name, t, T in (("A", A, contraction.A0), ("B", B, contraction.B0), ("C", C, contraction.C0)):
-            if (t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != T.size() or t.device.type != "cuda"
+            if (t.dtype != dt or not t.is_contiguous() or t.numel() != T.size() or t.device.type != "cuda"
                     or t.device != C.device):
-                raise ValueError("LoG: %s must be a contiguous float32 tensor of %d elements on C's GPU"
-                                 % (name, T.size()))
+                raise ValueError("LoG: %s must be a contiguous %s tensor of %d elements on C's GPU"
+                                 % (name, _TORCH_DTYPE[prec], T.size()))
         opA, opB = (B, A) if pb.swapped else (A, B)
-        lay = log_layout(pb, cand, shape)
+        lay = log_layout(pb, cand, prec, tile)
         self.layout = lay
-        chunks, offs, pos = [], [], 0
-        for arr in lay.tables:
-            a = np.ascontiguousarray(arr).view(np.int32)
-            offs.append(pos)
-            chunks.append(a)
-            pos += a.size
-            pad = (-pos) % 4   # every table 16-byte aligned
-            if pad:
-                chunks.append(np.zeros(pad, np.int32))
-                pos += pad
-        dev = C.device
-        self.tables = torch.from_numpy(np.concatenate(chunks)).to(dev)
-        tptr = [self.tables.data_ptr() + 4 * o for o in offs]
-        if PROBLEM_DTYPE.itemsize != ctypes.sizeof(_CProblem):
-            raise RuntimeError("LoG problem record layout mismatch")
-        tilesM, tilesN, _ = lay.tiles
-        nprob = len(lay.free_bases)
-        work = np.array([[p, tm, tn, 0] for p in range(nprob) for tm in range(tilesM) for tn in range(tilesN)],
-                        dtype=np.int32)
-        self.work = torch.from_numpy(work).to(dev)
-        self.nwork = int(work.shape[0])
-        self.grid = min(self.nwork, 512)
-        self.launches = [torch.from_numpy(r.view(np.uint8).copy()).to(dev)
-                         for r in problem_records(lay, opA.data_ptr(), opB.data_ptr(), C.data_ptr(), tptr,
-                                                  alpha, beta)]
+        offAm, offAk, offBn, offBk, offCm, offCn = lay.tables
+        esz = A.element_size()
+        g = GettGroup(prec, lay.tile, order)
+        for fa, fb, fc in lay.free_bases:
+            a = g.operand(opA.data_ptr() + esz * fa, offAm, offAk, lay.kfastA, False, key=("A", fa))
+            b = g.operand(opB.data_ptr() + esz * fb, offBn, offBk, lay.kfastB, True, key=("B", fb))
+            g.problem(a, b, C.data_ptr() + esz * fc, offCm, offCn, self.alpha, self.beta, locality=fa)
+        self.group = g.bind(C.device)
+        self.nlaunches = 2
         self._tensors = (A, B, C)
-        self.lib = load_library()
 
     def launch(self, stream=None) -> None:
-        import torch
-        from .runtime import GettRuntimeError
-        dev = self.tables.device
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
-        with torch.cuda.device(dev):
-            for probs in self.launches:
-                rc = self.lib.tccg_gett_batch_f32(ctypes.c_void_p(probs.data_ptr()),
-                                                  ctypes.c_void_p(self.work.data_ptr()), self.nwork, self.grid,
-                                                  ctypes.c_void_p(s.cuda_stream))
-                if rc != 0:
-                    raise GettRuntimeError("LoG launch failed: %s" % self.lib.tccg_last_error().decode())
+        self.group.launch(stream)
 
     def run(self, A=None, B=None, C=None, alpha=None, beta=None, stream=None) -> None:
-        """Launch on the bound tensors (the arguments exist for the timing
-        helpers' calling convention and must be the bound tensors)."""
+        """Launch on the bound tensors with the bound alpha and beta (the
+        arguments exist for the timing helpers' calling convention)."""
         for t, b in zip((A, B, C), self._tensors):
             if t is not None and t.data_ptr() != b.data_ptr():
                 raise ValueError("LoG runs on the tensors it was bound to")
+        if (alpha is not None and alpha != self.alpha) or (beta is not None and beta != self.beta):
+            raise ValueError("LoG runs with the alpha and beta it was bound to")
         self.launch(stream)

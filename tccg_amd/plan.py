@@ -763,27 +763,12 @@ def _c_segment_bytes(plan: GettPlan) -> float:
     return min(L, sub) * esz
 
 
-def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
-    """Estimate of one launch, the gfx950 analogue of Gett.estimateGFLOPS
-    (tccg/gett.py:590-687): per tile, MFMA time of the padded work plus a fixed
-    cost (gather latency and C write-back at the slot's share of HBM
-    bandwidth; mostly hidden by the persistent walk), times the number of
-    tile rounds over the CUs; bounded below by the HBM time of A, B and C."""
-    pb = plan.problem
+def _tile_costs(plan: GettPlan, beta: float = 0.0):
+    """Per-tile pieces of the launch estimate: (workgroup slots on the GPU,
+    MFMA time of one tile on one slot, its fixed cost, the pack kernels'
+    time, its C write-back time)."""
     hw = MI355X
     esz = 8 if plan.prec == F64 else 4
-    if plan.walk == 3:
-        # HBM time of A, B and C against the FMAs on the vector ALUs (one FMA
-        # per lane per cycle, half issued).  With four-wide groups C is written
-        # in whole-kilobyte runs, the best store pattern there is: no tile work
-        # beyond the HBM floor, so it wins ties there.  Scalar groups store one
-        # element per lane per instruction.
-        bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
-        eff = 0.8 if plan.modeA == FREE4 else 0.45
-        t_hbm = bytes_ / (eff * hw["hbm_bytes_per_s"])
-        t_valu = plan.padded_flops() / (0.5 * hw["valu"][plan.prec])
-        t = max(t_hbm, t_valu)
-        return t, (0.0 if plan.modeA == FREE4 else t)
     if plan.walk == 4:
         lds = sw_lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK)
     else:
@@ -791,7 +776,6 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
                else lds_bytes(plan.prec, plan.BM, plan.BN, plan.BK))
     per_cu = max(1, min(2, hw["lds_bytes_per_cu"] // lds))
     slots = hw["cus"] * per_cu
-    ntiles = plan.ntiles() * plan.ksplit
     eff = 0.75 if plan.BM * plan.BN >= 128 * 128 else 0.55
     t_pack = 0.0
     if plan.packed:
@@ -829,6 +813,45 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
         t_fixed = 0.1e-6 + t_store * 0.5
     elif plan.grid:
         t_fixed = 0.4e-6 + t_store * 0.6
+    return slots, t_mfma, t_fixed, t_pack, t_store
+
+
+def group_estimate(plans: Sequence[GettPlan], betas: Optional[Sequence[float]] = None) -> float:
+    """Estimate of one grouped launch over packed one-tile plans: every tile
+    of every plan shares the GPU's workgroup slots (no per-problem rounds),
+    plus one pack launch moving every operand once."""
+    betas = list(betas) if betas is not None else [0.0] * len(plans)
+    t, t_pack = 0.0, 0.0
+    for p, b in zip(plans, betas):
+        slots, t_mfma, t_fixed, tp, _ = _tile_costs(p, b)
+        t += p.ntiles() * (t_mfma + t_fixed) / slots
+        t_pack += max(0.0, tp - 6.0e-6)
+    return t + t_pack + 6.0e-6
+
+
+def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
+    """Estimate of one launch, the gfx950 analogue of Gett.estimateGFLOPS
+    (tccg/gett.py:590-687): per tile, MFMA time of the padded work plus a fixed
+    cost (gather latency and C write-back at the slot's share of HBM
+    bandwidth; mostly hidden by the persistent walk), times the number of
+    tile rounds over the CUs; bounded below by the HBM time of A, B and C."""
+    pb = plan.problem
+    hw = MI355X
+    esz = 8 if plan.prec == F64 else 4
+    if plan.walk == 3:
+        # HBM time of A, B and C against the FMAs on the vector ALUs (one FMA
+        # per lane per cycle, half issued).  With four-wide groups C is written
+        # in whole-kilobyte runs, the best store pattern there is: no tile work
+        # beyond the HBM floor, so it wins ties there.  Scalar groups store one
+        # element per lane per instruction.
+        bytes_ = esz * (pb.opA.size() + pb.opB.size() + pb.C.size() * (2 if beta != 0 else 1))
+        eff = 0.8 if plan.modeA == FREE4 else 0.45
+        t_hbm = bytes_ / (eff * hw["hbm_bytes_per_s"])
+        t_valu = plan.padded_flops() / (0.5 * hw["valu"][plan.prec])
+        t = max(t_hbm, t_valu)
+        return t, (0.0 if plan.modeA == FREE4 else t)
+    slots, t_mfma, t_fixed, t_pack, t_store = _tile_costs(plan, beta)
+    ntiles = plan.ntiles() * plan.ksplit
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
     if plan.walk == 4:
         # store waves write C while the compute waves run the next tile: the

@@ -46,7 +46,8 @@ EXPORTED_SYMBOLS = ["tccg_tile_shape", "tccg_num_tiles", "tccg_gett_f32", "tccg_
                     "tccg_gett_packed_bf16", "tccg_gett_packed_fp8", "tccg_gett_prepacked",
                     "tccg_num_vector_tiles", "tccg_vector_tile_shape",
                     "tccg_num_transpose_tiles", "tccg_transpose_tile_shape", "tccg_transpose_f32",
-                    "tccg_transpose_f64", "tccg_packed_sw_available", "tccg_gett_packed_noalloc"]
+                    "tccg_transpose_f64", "tccg_packed_sw_available", "tccg_gett_packed_noalloc",
+                    "tccg_gett_group"]
 
 _PACKED_FN = {F32: "tccg_gett_packed_f32", F64: "tccg_gett_packed_f64", BF16X3: "tccg_gett_packed_bf16x3",
               BF16: "tccg_gett_packed_bf16", FP8: "tccg_gett_packed_fp8"}
@@ -88,6 +89,9 @@ def load_library(path: Optional[Path] = None) -> ctypes.CDLL:
         lib.tccg_gett_prepacked.restype = ctypes.c_int
         lib.tccg_gett_packed_noalloc.argtypes = [ctypes.c_int, vp, vp, vp, f64, f64, ctypes.POINTER(_CPlan), vp]
         lib.tccg_gett_packed_noalloc.restype = ctypes.c_int
+        lib.tccg_gett_group.argtypes = [ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp,
+                                        ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_size_t, vp]
+        lib.tccg_gett_group.restype = ctypes.c_int
         lib.tccg_packed_workspace_bytes.argtypes = [ctypes.c_int, ctypes.POINTER(_CPlan)]
         lib.tccg_packed_workspace_bytes.restype = ctypes.c_size_t
         lib.tccg_packed_tile_shape.argtypes = lib.tccg_tile_shape.argtypes

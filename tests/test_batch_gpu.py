@@ -122,18 +122,27 @@ def _checksum(c, A, B):
     return float((ra * rb).sum()), float((np.abs(ra) * np.abs(rb)).sum())
 
 
-def test_full_batch_every_problem():
+@pytest.mark.parametrize("mode", ["persistent", "grouped"])
+def test_full_batch_every_problem(mode):
     """The benchmark's batch: 512 contractions of the 4 patterns, every extent
-    uniform over multiples of 8 in [8, 96] (seed 0), packed problems as in the
-    benchmark.  Every problem: against the FP64 einsum oracle on the CPU
-    (problems up to 2 GFLOP) or FP64 torch.einsum on the GPU (the rest, an
-    independent library path), and its element sum against the FP64 checksum."""
+    uniform over multiples of 8 in [8, 96] (seed 0), either with the large
+    problems on the packed path ahead of the persistent launch, or all of
+    them in one grouped packed launch (pack + GETT).  Every problem: against
+    the FP64 einsum oracle on the CPU (problems up to 2 GFLOP) or FP64
+    torch.einsum on the GPU (the rest, an independent library path), and its
+    element sum against the FP64 checksum."""
     import torch
+    from tccg_amd.plan import F32, PACKED_TILES
     cs = random_batch(512, seed=0)
     As, Bs, Cs, hA, hB = make_tensors(cs, 0)
-    b = GettBatch(cs, packed_min_tiles=512).bind(As, Bs, Cs)
-    print("full batch: %d problems, %d on the packed path, %d tiles" % (len(cs), len(b.large), b._bound["nwork"]),
-          flush=True)
+    if mode == "grouped":
+        b = GettBatch(cs, group_tile=PACKED_TILES[F32].index((128, 128, 16))).bind(As, Bs, Cs)
+        assert b.launches() == 2
+        print("full batch grouped: %d problems, %d tiles" % (len(cs), b._bound["group"]._bound["nwork"]), flush=True)
+    else:
+        b = GettBatch(cs, packed_min_tiles=512).bind(As, Bs, Cs)
+        print("full batch: %d problems, %d on the packed path, %d tiles" % (len(cs), len(b.large),
+                                                                            b._bound["nwork"]), flush=True)
     g = b.capture()
     g.replay()
     torch.cuda.synchronize()
@@ -151,7 +160,7 @@ def test_full_batch_every_problem():
             err = float((C.double() - ref).norm() / ref.norm())
             del ref
         worst = max(worst, err)
-        assert err < 1e-5, (i, i in b.large, str(c), err)
+        assert err < 1e-5, (i, mode, str(c), err)
         s, scale = _checksum(c, hA[i], hB[i])
         got = float(C.double().sum())
         assert abs(got - s) <= 1e-5 * scale, (i, str(c), got, s)

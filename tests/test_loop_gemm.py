@@ -1,14 +1,14 @@
 """LoG planning: the reference's candidate rules and selection, and a CPU
-model of the batched launches (shared GEMM tables, per-problem base offsets,
-accumulating contracted loops) checked against the FP64 einsum oracle, with
-every access asserted in bounds."""
+model of the grouped launch (shared GEMM tables, per-instance base offsets,
+contracted loops folded into the k range) checked against the FP64 einsum
+oracle, with every access asserted in bounds."""
 import numpy as np
 import pytest
 
 from oracle.einsum_ref import contract_ref, rel_frobenius
 from tccg_amd import Contraction
 from tccg_amd.loop_gemm import best_log_candidates, log_candidates, log_layout
-from tccg_amd.plan import FREE4, INVALID, KDIM4
+from tccg_amd.plan import BF16X3, F32, F64, INVALID, PACKED_TILES
 from tccg_amd.workloads import WORKLOADS, scaled
 
 
@@ -28,37 +28,35 @@ def test_known_candidates():
     assert [x.name for x in best] == ["LoG_free_dummy_contracted_dummy_batch_b_ack"]
 
 
-def emulate(c, cand, A, B, C, alpha, beta):
+def emulate(c, cand, A, B, C, alpha, beta, prec=F32):
+    """The grouped launch of the candidate: every instance (free loop
+    combination) C[ic] = alpha * A[ia] B[ib] + beta * C[ic] over the shared
+    tables, the k range folding the contracted loops."""
     pb = c.problem
-    lay = log_layout(pb, cand)
+    lay = log_layout(pb, cand, prec)
+    BM, BN, BK = PACKED_TILES[prec][lay.tile]
     opA, opB = (B, A) if pb.swapped else (A, B)
     a, b = np.asarray(opA, np.float64).reshape(-1), np.asarray(opB, np.float64).reshape(-1)
     out = np.asarray(C, np.float64).reshape(-1).copy()
-    offAm, offAk, offBn, offBk, offCm, offCn = (t.astype(np.int64) for t in lay.tables[:6])
+    offAm, offAk, offBn, offBk, offCm, offCn = (t.astype(np.int64) for t in lay.tables)
+    # table lengths the grouped kernels assume
+    assert offAm.size % BM == 0 and offAm.size % 64 == 0 and offBn.size % BN == 0 and offBn.size % 64 == 0
+    assert offAk.size == offBk.size and offAk.size % BK == 0 and offAk.size % 32 == 0
     vm, vn, vk = offAm != INVALID, offBn != INVALID, offAk != INVALID
     assert np.array_equal(vm, offCm != INVALID) and np.array_equal(vn, offCn != INVALID)
     assert np.array_equal(vk, offBk != INVALID)
     Am, Ak, Bn, Bk, Cm, Cn = offAm[vm], offAk[vk], offBn[vn], offBk[vk], offCm[vm], offCn[vn]
     written = np.zeros(out.size, dtype=np.int64)
-    for j, (ca, cb, _) in enumerate(lay.contracted_bases):
-        bj = beta if j == 0 else 1.0
-        for (fa, fb, fc) in lay.free_bases:
-            ia = fa + ca + Am[:, None] + Ak[None, :]
-            ib = fb + cb + Bk[:, None] + Bn[None, :]
-            ic = fc + Cm[:, None] + Cn[None, :]
-            assert ia.min() >= 0 and ia.max() < a.size and ib.max() < b.size and ic.max() < out.size
-            # four-wide load modes: groups of four consecutive, 16-byte aligned elements
-            for mode, idx, free_axis in ((lay.modeA, ia, 0), (lay.modeB, ib.T, 0)):
-                if mode in (FREE4, KDIM4):
-                    g = idx if mode == FREE4 else idx.T
-                    assert g.shape[0] % 4 == 0 and np.all(g[::4] % 4 == 0)
-                    for v in range(1, 4):
-                        assert np.array_equal(g[v::4], g[::4] + v)
-            prod = alpha * (a[ia] @ b[ib])
-            out[ic] = prod + (bj * out[ic] if bj != 0 else 0)
-            if j == 0:
-                written[ic.reshape(-1)] += 1
-    assert np.all(written == 1)     # every C element belongs to exactly one problem
+    for (fa, fb, fc) in lay.free_bases:
+        ia = fa + Am[:, None] + Ak[None, :]
+        ib = fb + Bk[:, None] + Bn[None, :]
+        ic = fc + Cm[:, None] + Cn[None, :]
+        assert ia.min() >= 0 and ia.max() < a.size and ib.min() >= 0 and ib.max() < b.size
+        assert ic.min() >= 0 and ic.max() < out.size
+        prod = alpha * (a[ia] @ b[ib])
+        out[ic] = prod + (beta * out[ic] if beta != 0 else 0)
+        written[ic.reshape(-1)] += 1
+    assert np.all(written == 1)     # every C element belongs to exactly one instance
     return out.reshape(np.shape(C))
 
 
@@ -80,25 +78,25 @@ def test_model_matches_oracle(wl, f):
         assert rel_frobenius(emulate(c, cand, A, B, C, 1.3, -0.7), ref) < 1e-13, cand.name
 
 
-def test_problem_records_match_layout():
-    """The batched launch's problem records: pointers are the bases the CPU
-    model checked, in FP32 elements, and the record layout is the C ABI's."""
-    import ctypes
-    from tccg_amd.batch import _CProblem
-    from tccg_amd.loop_gemm import PROBLEM_DTYPE, problem_records
-    assert PROBLEM_DTYPE.itemsize == ctypes.sizeof(_CProblem)
-    for name in PROBLEM_DTYPE.names:
-        assert PROBLEM_DTYPE.fields[name][1] == getattr(_CProblem, name).offset
+@pytest.mark.parametrize("prec", [F64, BF16X3])
+def test_model_other_precisions(prec):
+    """FP64 and split-BF16 take their own packed tile tables."""
     c = Contraction.from_tccg(scaled("abcd-aebf-dfce", 0.12))
+    rng = np.random.default_rng(3)
+    A, B, C = (rng.standard_normal(s) for s in c.shapes())
+    lab = lambda T: [i.label for i in T.indices]
+    ref = contract_ref(lab(c.C0), lab(c.A0), lab(c.B0), A, B, C, 0.5, 2.0)
     cand = log_candidates(c.problem)[0]
+    assert rel_frobenius(emulate(c, cand, A, B, C, 0.5, 2.0, prec), ref) < 1e-13
+
+
+def test_log_tile_and_sharing():
+    """Config (i)'s GEMM (72 x 72, k = 48 x 48 folded): 128 x 128 tiles, 72
+    packed A bases and 72 packed B bases shared by 5184 instances."""
+    from tccg_amd.loop_gemm import log_tile
+    c = Contraction.from_tccg(WORKLOADS["abcd-aebf-dfce"])
+    cand = best_log_candidates(log_candidates(c.problem), 1)[0]
     lay = log_layout(c.problem, cand)
-    recs = problem_records(lay, 1 << 20, 2 << 20, 3 << 20, list(range(8)), 2.0, 0.5)
-    assert len(recs) == len(lay.contracted_bases)
-    for j, r in enumerate(recs):
-        ca, cb, _ = lay.contracted_bases[j]
-        fb = np.array(lay.free_bases)
-        assert np.array_equal(r["A"], (1 << 20) + 4 * (fb[:, 0] + ca))
-        assert np.array_equal(r["B"], (2 << 20) + 4 * (fb[:, 1] + cb))
-        assert np.array_equal(r["C"], (3 << 20) + 4 * fb[:, 2])
-        assert np.all(r["beta"] == (0.5 if j == 0 else 1.0)) and np.all(r["alpha"] == 2.0)
-        assert np.all(r["tilesM"] == lay.tiles[0]) and np.all(r["offCn"] == 5)
+    assert PACKED_TILES[F32][lay.tile][:2] == (128, 128) and log_tile(F32, 72, 72) == lay.tile
+    assert lay.K == 48 * 48 and len(lay.free_bases) == 72 * 72
+    assert len({fa for fa, _, _ in lay.free_bases}) == 72 and len({fb for _, fb, _ in lay.free_bases}) == 72
