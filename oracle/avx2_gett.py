@@ -158,16 +158,19 @@ def _time(fn, budget: float, max_reps: int) -> tuple:
 def bench_workload(name: str, seconds: float = 5.0, top: int = 3, log=print) -> dict:
     """The reference's measure-and-select on the host (tccg/tccg.py:987-1058,
     timing by min over repetitions, tccg/tccg.py:725-737): the restated
-    planner's `top` best-ranked candidates, each timed on all physical cores,
-    the fastest kept and also timed on one core; its result checked against
-    the FP64 einsum oracle (relative Frobenius error <= 1e-5)."""
-    from tccg_amd.frontend import parse_tccg  # parser only: labels and sizes
-    from tccg_amd.workloads import get
+    planner's `top` best-ranked candidates, each emitted as its own
+    per-contraction source and compiled (oracle/avx2_emit.py, as the
+    reference generates and compiles one function per candidate), timed on
+    all physical cores, the fastest kept and also timed on one core; its
+    result checked against the FP64 einsum oracle (relative Frobenius error
+    <= 1e-5) and against the table-driven restatement (same bits)."""
+    from .avx2_emit import EmittedGett, workload_labels
     from .avx2_model import candidates_for
     from .einsum_ref import contract_ref, rel_frobenius
-    C_, A_, B_, alpha, beta = parse_tccg(get(name))
-    c, a, b = ([i.label for i in T.indices] for T in (C_, A_, B_))
-    sizes = {i.label: i.size for T in (C_, A_, B_) for i in T.indices}
+    from tccg_amd.frontend import parse_tccg  # parser only: alpha and beta
+    from tccg_amd.workloads import get
+    _, _, _, alpha, beta = parse_tccg(get(name))
+    c, a, b, sizes = workload_labels(name)
     ranked = candidates_for(c, a, b, sizes, "float", beta, 16)
     rng = np.random.default_rng(0)
     A = rng.uniform(-1, 1, [sizes[l] for l in reversed(a)]).astype(np.float32)
@@ -176,9 +179,9 @@ def bench_workload(name: str, seconds: float = 5.0, top: int = 3, log=print) -> 
     cores = physical_cores()
     timed = []
     for cand in ranked:
-        g = Avx2Gett(c, a, b, sizes, candidate=cand)
-        if not g.supported():
+        if not Avx2Gett(c, a, b, sizes, candidate=cand).supported():
             continue
+        g = EmittedGett(c, a, b, sizes, cand)
         t, reps = _time(lambda: g(A, B, C, alpha, beta, threads=cores), seconds * 0.5 / top, 20)
         timed.append((t, reps, g, cand))
         log("cpu baseline %s %s: estimated %.1f GFLOP/s (one Haswell core), %d threads %.1f GFLOP/s"
@@ -193,10 +196,15 @@ def bench_workload(name: str, seconds: float = 5.0, top: int = 3, log=print) -> 
     g(A, B, C, alpha, beta, threads=cores)
     err = rel_frobenius(C, contract_ref(c, a, b, A, B, None, alpha, 0.0))
     if not err <= 1e-5:
-        raise RuntimeError("restated AVX2 GETT misses the einsum oracle on %s: %.3e" % (name, err))
+        raise RuntimeError("emitted AVX2 GETT misses the einsum oracle on %s: %.3e" % (name, err))
+    C2 = np.zeros_like(C)
+    Avx2Gett(c, a, b, sizes, candidate=cand)(A, B, C2, alpha, beta, threads=cores)
+    if not np.array_equal(C, C2):
+        raise RuntimeError("emitted and table-driven AVX2 GETT differ on %s" % name)
     gf_all, gf_one = g.flops() / t_all / 1e9, g.flops() / t_one / 1e9
     log("cpu baseline %s: %s, %d threads %.1f GFLOP/s, 1 thread %.1f GFLOP/s, rel. err %.1e"
         % (name, cand.name()[:60], cores, gf_all, gf_one, err))
-    return {"workload": name, "candidate": cand.name(), "estimated_gflops_one_haswell_core": round(cand.estimate, 2),
+    return {"workload": name, "candidate": cand.name(), "code": "emitted per contraction",
+            "estimated_gflops_one_haswell_core": round(cand.estimate, 2),
             "gflops": round(gf_all, 1), "gflops_one_core": round(gf_one, 1), "cores": cores,
             "candidates_timed": len(timed), "reps": reps_all, "reps_one_core": reps_one, "rel_err_vs_fp64": err}
