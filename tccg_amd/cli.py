@@ -18,8 +18,9 @@ at most --maxImplementationsLoG / --maxImplementationsTTGT of each (default 1,
 as in the reference), and reported as the reference's harness reports them
 ("loopGemm_<name>: X GFLOPS", "ttgemmt_<name>: X GFLOPS", tccg/tccg.py:1060-1174;
 "Best loop-based / TTGT-based implementation ...", tccg/tccg.py:377, 399).
-The emitted entry point is always the fastest GETT: the drop-in boundary is
-the generated gett().  The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL:
+When LoG or TTGT beats the fastest GETT, its code is emitted instead, as the
+reference does (loopOverGemm.hip / ttgemmt.hip with the reference's method
+entry point beside the same gett() boundary, tccg_amd.codegen_methods).  The summary line "Best Loop/TTGT/GETT/reference/GEMM/TCL:
 ..." is printed as the reference does (tccg/tccg.py:426); the GEMM and TCL
 methods are not part of this backend and report 0.00 / -1.
 
@@ -170,8 +171,8 @@ def main(argv=None) -> int:
     chosen = plans[0]
     best_gflops = -1.0
     ref_gflops = -1.0
-    ttgt_gflops = 0.0
-    log_gflops = 0.0
+    ttgt_gflops, ttgt_cand, ttgt_plan = 0.0, None, None
+    log_gflops, log_cand = 0.0, None
     gpu = False
     if not args.generateOnly:
         try:
@@ -199,22 +200,34 @@ def main(argv=None) -> int:
             if chosen is None:
                 return 1
             if not args.noLoG:
-                log_gflops = _time_log(args, contraction, prec)
+                log_gflops, log_cand = _time_log(args, contraction, prec)
                 if log_gflops is None:
                     return 1
             if not args.noTTGT:
-                ttgt_gflops = _time_ttgt(args, contraction, prec)
+                ttgt_gflops, ttgt_cand, ttgt_plan = _time_ttgt(args, contraction, prec)
                 if ttgt_gflops is None:
                     return 1
-            other = max(log_gflops, ttgt_gflops)
-            if other > best_gflops:
-                print(WARNING + "%s is faster for this contraction; the emitted entry point stays the fastest GETT"
-                      % ("LoG" if log_gflops >= ttgt_gflops else "TTGT") + ENDC)
         db.close()
-    print(OKGREEN + "[generate] emitting %s" % candidate_name(chosen) + ENDC)
+    # emitFinalCode (tccg/tccg.py:158-186): the fastest method's code; GETT
+    # wins ties, then LoG over TTGT
+    method = "gett"
+    if prec in (F32, F64) and best_gflops > 0:
+        if log_cand is not None and log_gflops > best_gflops and log_gflops >= ttgt_gflops:
+            method = "log"
+        elif ttgt_cand is not None and ttgt_gflops > best_gflops:
+            method = "ttgt"
     try:
-        from . import codegen
-        res = codegen.generate(chosen, text, out_dir, "gett")
+        if method == "gett":
+            print(OKGREEN + "[generate] emitting %s" % candidate_name(chosen) + ENDC)
+            from . import codegen
+            res = codegen.generate(chosen, text, out_dir, "gett")
+        else:
+            from . import codegen_methods
+            cand = log_cand if method == "log" else ttgt_cand
+            label = ("loopGemm_" if method == "log" else "ttgemmt_") + cand.name
+            print(OKGREEN + "[generate] emitting %s (faster than the best GETT)" % label + ENDC)
+            res = codegen_methods.generate_method(method, contraction, cand, prec, text, out_dir,
+                                                  gemm_plan=ttgt_plan if method == "ttgt" else None)
     except Exception as e:
         print(FAIL + "[TCC Error] compilation failed: %s" % e + ENDC, file=sys.stderr)
         return 1
@@ -289,13 +302,13 @@ def _time_log(args, contraction, prec):
     cands = log_candidates(contraction.problem)
     if cands is None or prec not in GROUP_PRECS:
         print("LoG not possible" if cands is None else "LoG: not run in %s" % PREC_NAMES[prec])
-        return 0.0
+        return 0.0, None
     best = best_log_candidates(cands, args.maxImplementationsLoG)
     print("Total amount of LoG implementations: %d" % len(cands))
     print("Best LoG candidates: %s" % ", ".join(c.name for c in best))
     A, B, C0, ref = _test_inputs(contraction, prec, args.testing)
     C = C0.clone()
-    top, top_name = -1.0, ""
+    top, top_name, top_cand = -1.0, "", None
     for cand in best:
         d = DeviceLoG(contraction, cand, A, B, C, prec=prec)
         name = "loopGemm_" + cand.name
@@ -305,15 +318,15 @@ def _time_log(args, contraction, prec):
             torch.cuda.synchronize()
             if not _check_result(name, prec, C, ref, 1e-5):
                 print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
-                return None
+                return None, None
         C.copy_(C0)
         t = time_plan(d, A, B, C, contraction.alpha, contraction.beta, reps=2 if args.fastMeasurements else 5)
         gf = contraction.flops() / t / 1e9
         print("%s: %.2f GFLOPS" % (name, gf))
         if gf > top:
-            top, top_name = gf, name
+            top, top_name, top_cand = gf, name, cand
     print("Best loop-based implementation (%s) attained: %.2f GFLOPS/s" % (top_name, max(top, 0.0)))
-    return top
+    return top, top_cand
 
 
 def _time_ttgt(args, contraction, prec):
@@ -331,7 +344,7 @@ def _time_ttgt(args, contraction, prec):
     A, B, C0, ref = _test_inputs(contraction, prec, args.testing)
     C = C0.clone()
     tol = {F32: 1e-5, F64: 1e-13, BF16X3: 1e-5, FP8: 8e-2}.get(prec, 1e-2)
-    top, top_name = -1.0, ""
+    top, top_name, top_cand, top_plan = -1.0, "", None, None
     for cand in best:
         dt_ = DeviceTtgt(contraction, cand, prec, tune=True)
         name = "ttgemmt_" + cand.name
@@ -341,16 +354,16 @@ def _time_ttgt(args, contraction, prec):
             torch.cuda.synchronize()
             if not _check_result(name, prec, C, ref, tol):
                 print(FAIL + "[TCC Error] runtime error." + ENDC, file=sys.stderr)
-                return None
+                return None, None, None
         C.copy_(C0)
         t = time_plan(dt_, A, B, C, contraction.alpha, contraction.beta, reps=2 if args.fastMeasurements else 5)
         gf = contraction.flops() / t / 1e9
         print("%s: %.2f GFLOPS" % (name, gf))
         if gf > top:
-            top, top_name = gf, name
+            top, top_name, top_cand, top_plan = gf, name, cand, dt_.gemm_plan
         del dt_
     print("Best TTGT-based implementation (%s) attained: %.2f GFLOPS/s" % (top_name, max(top, 0.0)))
-    return top
+    return top, top_cand, top_plan
 
 
 def _time_candidates(args, contraction, prec, plans, db, device):
