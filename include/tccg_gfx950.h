@@ -106,7 +106,16 @@ typedef struct tccg_gett_plan {
                        /* computing tiles through the LDS copy ring and     */
                        /* four waves writing the finished tiles to C from   */
                        /* LDS, so that C's write-back overlaps the next     */
-                       /* tiles' compute; grid > 0                          */
+                       /* tiles' compute; grid > 0;                         */
+                       /* 5: packed plans only, the stream-k walk over      */
+                       /* `grid` workgroups (grid > 0, ksplit 0 or 1): all  */
+                       /* but the last one or two rounds of tiles walked    */
+                       /* whole, the k-steps of the rest split evenly over  */
+                       /* the grid; a tile cut by the split is finished by  */
+                       /* the workgroup holding its last k-step, which adds */
+                       /* the other pieces' sums from the workspace (fixed  */
+                       /* order: the same bits from call to call) and       */
+                       /* writes C once; run epilogue allowed               */
   int32_t crun_m;      /* packed plans, one tile per workgroup: when > 0,   */
   int32_t crun_n;      /* C's stride-one index spans crun_m consecutive m   */
                        /* positions and C's next index crun_n consecutive n */
@@ -181,7 +190,10 @@ int tccg_gett_bf16(const float* A, const float* B, float* C, float alpha, float 
  * owned by the caller and not shared by concurrent calls; the figure ends
  * with 160 KiB that the GETT kernel's prefetch may read past the operands
  * (never writes), so the whole span must be readable device memory.  TCCG_FP8 reads the
- * plan's amax as tccg_gett_fp8 does.
+ * plan's amax as tccg_gett_fp8 does.  A stream-k plan (walk 5) also keeps
+ * one partial tile per workgroup and grid + 1 flag words there; a packing
+ * call clears the flags before its GETT kernel and every call leaves them
+ * clear, so tccg_gett_prepacked() needs no clearing.
  */
 size_t tccg_packed_workspace_bytes(int prec, const tccg_gett_plan* plan);
 /* 1 when the store-wave kernel (walk 4) is built for packed tile `tile` of

@@ -54,14 +54,27 @@ def _table_names(name: str):
     return {k: "%s_%s" % (base, k) for k in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB")}
 
 
-def _part_expr(plan: GettPlan, T: str) -> str:
-    """The partial-tile buffer of a split-k plan: the workspace after the
-    packed operands, at a 256-byte boundary (tccg_packed_workspace_bytes)."""
-    if plan.ksplit <= 1:
-        return "nullptr"
+def _part_offset(plan: GettPlan) -> int:
+    """Workspace offset of the partial tiles (split-k and stream-k plans):
+    after the packed operands, at a 256-byte boundary
+    (tccg_packed_workspace_bytes)."""
     per = {F32: 4, F64: 8, BF16X3: 6, BF16: 2, FP8: 1}[plan.prec] * plan.Kp
     ab = -(-(per * plan.Mp) // 256) * 256 + per * plan.Np
-    return "reinterpret_cast<%s*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % (T, -(-ab // 256) * 256)
+    return -(-ab // 256) * 256
+
+
+def _part_expr(plan: GettPlan, T: str) -> str:
+    """The partial-tile buffer of a split-k or stream-k plan."""
+    if plan.ksplit <= 1 and plan.walk != 5:
+        return "nullptr"
+    return "reinterpret_cast<%s*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % (T, _part_offset(plan))
+
+
+def _flags_offset(plan: GettPlan) -> int:
+    """Workspace offset of a stream-k plan's flag words: after one partial
+    tile per workgroup, at a 256-byte boundary."""
+    esz = 8 if plan.prec == F64 else 4
+    return _part_offset(plan) + -(-(plan.grid * plan.BM * plan.BN * esz) // 256) * 256
 
 
 def _reduce_launch(plan: GettPlan, T: str) -> str:
@@ -93,6 +106,11 @@ def emit_source(plan: GettPlan, contraction_text: str, name: str = "gett") -> st
         kern = "tccg::gett_packed_sw_kernel<%s, %d, %d, %d, %d>" % (
             _PREC_ENUM[plan.prec], plan.BM, plan.BN, plan.BK, stages)
         shape = "tccg::SwShape<%s, %d, %d, %d, %d>" % (_PREC_ENUM[plan.prec], plan.BM, plan.BN, plan.BK, stages)
+    elif plan.packed and plan.walk == 5:
+        WM, WN = 2, 2
+        kern = "tccg::gett_packed_sk_kernel<%s, %d, %d, %d, 2, 2, 3>" % (
+            _PREC_ENUM[plan.prec], plan.BM, plan.BN, plan.BK)
+        shape = "tccg::PackedShape<%s, %d, %d, %d, 3>" % (_PREC_ENUM[plan.prec], plan.BM, plan.BN, plan.BK)
     elif plan.packed:
         WM, WN = 2, 2
         kern = "tccg::gett_packed_kernel<%s, %d, %d, %d, 2, 2, 3, %s>" % (
@@ -272,8 +290,8 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
   p.alpha = alpha; p.beta = beta;
   p.amax = %s;
   p.crunM = %d; p.crunN = %d; p.cvec = %d;
-  p.ksplit = %d; p.part = %s;
-  hipLaunchKernelGGL(kern, dim3(%d), dim3(%d), kLds, s, p);
+  p.ksplit = %d; p.part = %s; p.flags = %s;
+%s  hipLaunchKernelGGL(kern, dim3(%d), dim3(%d), kLds, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
 %s  return 0;
@@ -282,6 +300,11 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
 """ % (T, plan.Mp // plan.BM, plan.Np // plan.BN, plan.Kp // plan.BK, plan.groupM,
        "t->amax" if plan.prec == FP8 else "nullptr", plan.crun[0], plan.crun[1], plan.cvec,
        plan.ksplit, _part_expr(plan, T),
+       ("reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % _flags_offset(plan)
+        if plan.walk == 5 else "nullptr"),
+       ("  // the stream-k flags start clear\n"
+        "  hipLaunchKernelGGL(tccg::sk_clear_kernel<uint32_t>, dim3(%d), dim3(256), 0, s, p.flags, %d);\n"
+        % (-(-(plan.grid + 1) // 256), plan.grid + 1) if plan.walk == 5 else ""),
        plan.grid if plan.grid else plan.ntiles() * plan.ksplit, 512 if plan.walk == 4 else 256,
        _reduce_launch(plan, T)))
     out.append("void gett({T}* A, {T}* B, {T}* C, const {T} alpha, const {T} beta) {{\n"
@@ -356,6 +379,8 @@ def packed_suffix(packed) -> str:
         return "_storewaves"
     if packed.get("ksplit", 1) > 1:
         return "_ksplit%d" % packed["ksplit"]
+    if packed.get("stream_k"):
+        return "_streamk%d" % packed["stream_k"]
     return "_packed"
 
 
@@ -381,13 +406,14 @@ def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) 
     if workloads is None:
         # packed-operand libraries: split-BF16 on the compute-bound contraction,
         # FP8 (persistent walk) on the precision contraction, FP8 store waves,
-        # and a split-k plan on the first adjoint of abcd-aklb-lckd (small M*N,
-        # long k)
+        # and split-k and stream-k plans on the first adjoint of abcd-aklb-lckd
+        # (small M*N, long k)
         texts.update(SPLIT_K_TEXTS)
         todo += [("abcd-aebf-dfce", BF16X3, dict(tile=0, persistent=False)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=0, persistent=True)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=4, store_waves=True)),
-                 ("aklb-abcd-lckd", F32, dict(tile=0, ksplit=4))]
+                 ("aklb-abcd-lckd", F32, dict(tile=0, ksplit=4)),
+                 ("aklb-abcd-lckd", F32, dict(tile=2, stream_k=1))]
     for wl, prec, packed in todo:
         text = texts[wl]
         c = Contraction.from_tccg(text)

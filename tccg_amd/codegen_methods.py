@@ -187,7 +187,7 @@ def emit_ttgt_source(contraction, cand, prec: int, gemm_plan, contraction_text: 
         raise ValueError("the TTGT GEMM plan must be a %s plan" % PREC_NAMES[prec])
     gt = {k: blob.add(getattr(g, k)) for k in ("offAm", "offAk", "offBn", "offBk", "offCm", "offCn", "ordA", "ordB")}
     ws_gemm = g.workspace_bytes() if g.packed else 0
-    crun = (g.crun[0], 0) if g.walk == 4 else (g.crun if not g.grid else (0, 0))
+    crun = (g.crun[0], 0) if g.walk == 4 else (g.crun if not g.grid or g.walk == 5 else (0, 0))
     fn = {F32: "f32", F64: "f64"}[prec]
     gemm_call = ("tccg_gett_packed_%s(gA, gB, out, alpha_, beta_, &gp, st->ws, %d, s)" % (fn, ws_gemm) if g.packed
                  else "tccg_gett_%s(gA, gB, out, alpha_, beta_, &gp, s)" % fn)

@@ -27,7 +27,7 @@ gett_batch_kernel_f32(const tccg_batch_problem* __restrict__ probs, const tccg_b
     p.offCm = q.offCm; p.offCn = q.offCn; p.ordA = q.ordA; p.ordB = q.ordB;
     p.tilesM = q.tilesM; p.tilesN = q.tilesN; p.kTiles = q.kTiles; p.groupM = 1;
     p.alpha = q.alpha; p.beta = q.beta; p.amax = nullptr;
-    p.crunM = 0; p.crunN = 0; p.cvec = 0; p.ksplit = 1; p.part = nullptr;
+    p.crunM = 0; p.crunN = 0; p.cvec = 0; p.ksplit = 1; p.part = nullptr; p.flags = nullptr;
     switch (q.variant) {
 #define TCCG_V(SH, MA, MB) case SH * 16 + MA * 4 + MB: batch_tile<SH, MA, MB>(p, wk.tm, wk.tn, smem); break;
 #define TCCG_VB(SH, MA) TCCG_V(SH, MA, 0) TCCG_V(SH, MA, 1) TCCG_V(SH, MA, 2) TCCG_V(SH, MA, 3)

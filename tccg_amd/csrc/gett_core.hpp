@@ -100,6 +100,11 @@ struct GettParams {
   // parts and applies alpha and beta.
   int ksplit;
   TG* part;
+  // Stream-k walk (gett_packed_sk_kernel): part holds one partial tile per
+  // workgroup (BM * BN elements, in accumulator register order), flags one
+  // word per workgroup (1: that workgroup's partial is complete) and, after
+  // them, an error word (bit 0: a wait for a partial gave up).
+  uint32_t* flags;
 };
 
 template <int P> struct PrecTraits;

@@ -67,7 +67,7 @@ gett_group_kernel(const tccg_group_problem* __restrict__ probs, int nprobs, cons
   p.tilesM = q.tilesM; p.tilesN = q.tilesN; p.kTiles = q.kTiles; p.groupM = 1;
   p.alpha = (TG)q.alpha; p.beta = (TG)q.beta;
   p.amax = nullptr;
-  p.crunM = 0; p.crunN = 0; p.cvec = 0; p.ksplit = 1; p.part = nullptr;
+  p.crunM = 0; p.crunN = 0; p.cvec = 0; p.ksplit = 1; p.part = nullptr; p.flags = nullptr;
   packed_walk<P, BM, BN, BK, WM, WN, S, false>(p, v, nwork, wk.tm, wk.tn, 0, 0, q.kTiles, smem_raw);
 }
 

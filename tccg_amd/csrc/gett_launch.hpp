@@ -272,6 +272,7 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
   prm.cvec = 0;
   prm.ksplit = 1;
   prm.part = nullptr;
+  prm.flags = nullptr;
   if (P == tccg::FP8 && !plan->amax) return fail(1, "FP8 plan needs the device amax pointer");
   const int64_t nblocks = (int64_t)prm.tilesM * prm.tilesN;
   if (nblocks > 0x7fffffff) return fail(1, "too many tiles");
@@ -302,10 +303,13 @@ int run_gett(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::Pre
 
 // Bytes of a packed plan's workspace: A's planes, padded to 256 bytes, B's
 // planes, with ksplit > 1 (padded to 256 bytes) the ksplit partial C tiles
-// of every tile (FP64 for FP64, else FP32), then kPackedTailBytes that the
-// kernel's prefetch may read (never write).  0 if the plan's shape is not
-// one of the packed tiles.
-inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB, size_t* offP = nullptr) {
+// of every tile (FP64 for FP64, else FP32), with the stream-k walk (walk 5;
+// padded to 256 bytes) one partial tile per workgroup, then (padded to 256
+// bytes) one flag word per workgroup and the error word, then
+// kPackedTailBytes that the kernel's prefetch may read (never write).  0 if
+// the plan's shape is not one of the packed tiles.
+inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_t* offB, size_t* offP = nullptr,
+                                     size_t* offF = nullptr) {
   int ntiles = 0;
   const TileCfg* tiles = packed_tiles_of(prec, &ntiles);
   if (!tiles || !plan || plan->tile < 0 || plan->tile >= ntiles || plan->Mp <= 0 || plan->Np <= 0 || plan->Kp <= 0)
@@ -323,6 +327,12 @@ inline size_t packed_workspace_bytes(int prec, const tccg_gett_plan* plan, size_
   if (offB) *offB = a;
   const size_t ab = (a + b + 255) & ~(size_t)255;
   if (offP) *offP = ab;
+  if (plan->walk == 5 && plan->grid > 0) {
+    const TileCfg& tc = tiles[plan->tile];
+    const size_t parts = ((size_t)plan->grid * tc.BM * tc.BN * (prec == tccg::F64 ? 8 : 4) + 255) & ~(size_t)255;
+    if (offF) *offF = ab + parts;
+    return ab + parts + ((((size_t)plan->grid + 1) * 4 + 255) & ~(size_t)255) + tccg::kPackedTailBytes;
+  }
   if (plan->ksplit > 1)
     return ab + (size_t)plan->ksplit * (size_t)plan->Mp * (size_t)plan->Np * (prec == tccg::F64 ? 8 : 4) +
            tccg::kPackedTailBytes;
@@ -423,6 +433,45 @@ int dispatch_packed_sw_tile(int tile, const tccg::GettParams<typename tccg::Prec
       return fail(1, "the store-wave kernel (walk 4) is not built for this precision and tile");
     }
     return dispatch_packed_sw_tile<P, I + 1>(tile, prm, grid, s);
+  }
+}
+
+// Stream-k walk (walk 5) over `grid` workgroups.  The grid is not capped at
+// the resident limit: a workgroup only waits for workgroups of lower index,
+// which were dispatched before it.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+int launch_packed_sk_kernel(const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
+                            hipStream_t stream) {
+  constexpr int ST = kPackedStages;
+  using SH = tccg::PackedShape<P, BM, BN, BK, ST>;
+  auto kern = tccg::gett_packed_sk_kernel<P, BM, BN, BK, WM, WN, ST>;
+  static std::atomic<uint64_t> lds_set{0};
+  if (int rc = ensure_dynamic_lds(reinterpret_cast<const void*>(kern), SH::LDS_BYTES, lds_set)) return rc;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), SH::LDS_BYTES, stream, prm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+template <int P, int I = 0>
+int dispatch_packed_sk_tile(int tile, const tccg::GettParams<typename tccg::PrecTraits<P>::TG>& prm, int grid,
+                            hipStream_t s) {
+  if constexpr (I >= packed_count(P)) {
+    return fail(1, "invalid packed tile id");
+  } else {
+    constexpr TileCfg c = packed_tile(P, I);
+    if (tile == I) {
+      if (prm.crunM > 0) {
+        using TG = typename tccg::PrecTraits<P>::TG;
+        constexpr int VEC = 16 / (int)sizeof(TG);
+        constexpr int rows_max = tccg::PackedShape<P, c.BM, c.BN, c.BK, kPackedStages>::LDS_BYTES /
+                                 ((c.BM + VEC) * (int)sizeof(TG));
+        if (prm.crunM % VEC || c.BM % prm.crunM || c.BN % prm.crunN || rows_max < prm.crunN)
+          return fail(1, "crun_m/crun_n do not fit the tile (whole runs, 16-byte vectors, LDS rows)");
+      }
+      return launch_packed_sk_kernel<P, c.BM, c.BN, c.BK, c.WM, c.WN>(prm, grid, s);
+    }
+    return dispatch_packed_sk_tile<P, I + 1>(tile, prm, grid, s);
   }
 }
 
@@ -528,8 +577,8 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   if (plan->Mp % tc.BM || plan->Np % tc.BN || plan->Kp % tc.BK)
     return fail(1, "table lengths must be whole multiples of the tile shape");
   if (plan->Kp % tccg::kPackTK) return fail(1, "packed plans need Kp to be a multiple of 32");
-  size_t offB = 0, offP = 0;
-  const size_t need = packed_workspace_bytes(P, plan, &offB, &offP);
+  size_t offB = 0, offP = 0, offF = 0;
+  const size_t need = packed_workspace_bytes(P, plan, &offB, &offP, &offF);
   if (workspace_bytes < need) return fail(1, "workspace too small for the packed operands");
   // the packed kernels address elements with 32-bit offsets
   if ((offP - offB) / sizeof(TL) >= 0xffffffffull || offB / sizeof(TL) >= 0xffffffffull)
@@ -558,6 +607,8 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   prm.cvec = plan->cvec;
   prm.ksplit = plan->ksplit > 1 ? plan->ksplit : 1;
   prm.part = reinterpret_cast<TG*>(reinterpret_cast<unsigned char*>(workspace) + offP);
+  prm.flags = plan->walk == 5 ? reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(workspace) + offF)
+                              : nullptr;
   if (plan->ksplit < 0) return fail(1, "ksplit must be >= 0");
   if (prm.ksplit > 1) {
     if (plan->walk != 0 || (plan->grid > 0 && plan->grid < (int64_t)prm.tilesM * prm.tilesN))
@@ -578,7 +629,12 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
     if ((plan->cvec || plan->crun_m) && !aligned16(C)) return fail(1, "16-byte stores need a 16-byte aligned C");
     if (!std::is_same<TG, float>::value) return fail(1, "the store-wave kernel (walk 4) writes FP32 results only");
   } else {
-    if (plan->walk != 0) return fail(1, "packed plans take walk 0 or 4");
+    if (plan->walk != 0 && plan->walk != 5) return fail(1, "packed plans take walk 0, 4 or 5");
+    if (plan->walk == 5) {
+      if (plan->grid <= 0) return fail(1, "the stream-k walk (walk 5) needs grid > 0");
+      if (prm.ksplit > 1) return fail(1, "the stream-k walk (walk 5) takes ksplit 0 or 1");
+      if ((int64_t)nblocks * prm.kTiles >= (int64_t)1 << 50) return fail(1, "too many k-steps");
+    }
     if (plan->crun_m < 0 || plan->crun_n < 0 || (plan->crun_m > 0) != (plan->crun_n > 0))
       return fail(1, "crun_m and crun_n must both be positive or both zero");
     if (plan->crun_m > 0 && !aligned16(C))
@@ -590,6 +646,15 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
   }
   if (plan->walk == 4) {
     if constexpr (std::is_same<TG, float>::value) return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
+  }
+  if (plan->walk == 5) {
+    if (pack) {
+      hipLaunchKernelGGL(tccg::sk_clear_kernel<uint32_t>, dim3((plan->grid + 1 + 255) / 256), dim3(256), 0, s, prm.flags,
+                         plan->grid + 1);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return fail((int)e, std::string("flag clear launch: ") + hipGetErrorString(e));
+    }
+    return dispatch_packed_sk_tile<P>(plan->tile, prm, plan->grid, s);
   }
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
 }

@@ -298,13 +298,163 @@ __device__ inline void store_tile_staged(const GettParams<typename PrecTraits<P>
 // blockIdx.x walks tiles blockIdx.x, + gridDim.x, ...; the copies and the
 // fragment reads run on across tiles, so the next tile's first k-steps are
 // in flight during the epilogue.
+
+// One piece of a stream-k segment (gett_packed_sk_kernel): k-steps
+// [k0, k0 + KT) of one tile.  The tiles of XCD group `xcd` are split into
+// `nseg` consecutive segments of its `units` k-steps (tile-major), segment j
+// belonging to workgroup xcd + nx j; a tile cut by segment boundaries is
+// finished by the segment that holds its last k-step.
+struct SkPiece {
+  int produce;      // the tile's later k-steps belong to the next segment: publish the raw sums
+  int combine;      // the tile's earlier k-steps belong to earlier segments: add their sums, then store
+  int slot;         // this workgroup's partial slot and flag (its workgroup index)
+  int nx;           // XCD groups (8, fewer for a grid under 8)
+  int xcd, seg, nseg;
+  long long units;  // k-steps of the XCD's tiles
+  long long first;  // the tile's first k-step among them
+};
+
+// Spin limit of a combining workgroup's wait for one partial (each round
+// sleeps about 128 cycles): about 0.2 s, far beyond any producer's lead.
+constexpr int kSkSpinLimit = 1 << 21;
+
+// Partial slot layout: 16-byte vectors of VEC consecutive accumulator
+// elements in register order; vector q of wave w and lane l at byte
+// ((w * NV + q) * 64 + l) * 16, written and read by the same wave and lane
+// of two workgroups running the same tile shape, so every access is a
+// contiguous kilobyte per wave.
+//
+// Visibility without cache-wide maintenance: the producer writes its partial
+// with device-coherent (sc1) stores, which go through its XCD's L2 to the
+// device's point of coherence, waits until they have completed (vmcnt)
+// before the barrier after which its flag is stored (sc1), and stores
+// 1 + its XCD id as the flag.  A reader on the same XCD then loads the
+// partial past its own L1 (sc0) from the shared L2, which the stores went
+// through; a reader on another XCD loads it device-coherently (sc1) from
+// where the stores completed.  Neither needs the L2 writeback of a release
+// fence or the L2 invalidation of an acquire fence, which would stall every
+// CU of the XCD.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+struct SkSlot {
+  using TS = TileShape<P, BM, BN, BK, WM, WN>;
+  using TG = typename PrecTraits<P>::TG;
+  static constexpr int VEC = 16 / (int)sizeof(TG);
+  static constexpr int NACC = Mma<P>::NACC;
+  static_assert(NACC % VEC == 0, "whole 16-byte vectors per accumulator");
+  static constexpr int NV = TS::SN * TS::SM * NACC / VEC;   // vectors per lane
+  static_assert(4 * NV * 64 * 16 == BM * BN * (int)sizeof(TG), "a slot holds one tile");
+  union V {
+    f32x4 bits;
+    TG e[VEC];
+  };
+  __device__ static __amdgpu_buffer_rsrc_t rsrc(const GettParams<TG>& p, int slot) {
+    return __builtin_amdgcn_make_buffer_rsrc(p.part + (size_t)slot * (BM * BN), 0, BM * BN * (int)sizeof(TG),
+                                             0x00020000);
+  }
+};
+
+__device__ inline uint32_t xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 0xfu; }
+
+// acc += the partial in slot `slot`, loaded with cache policy CPOL (1: past
+// the L1, from the L2; 16: device-coherent).
+template <int P, int BM, int BN, int BK, int WM, int WN, int CPOL>
+__device__ inline void sk_add_partial(const GettParams<typename PrecTraits<P>::TG>& p,
+                                      typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                                 [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                      int slot, int w, int lane) {
+  using TS = TileShape<P, BM, BN, BK, WM, WN>;
+  using SL = SkSlot<P, BM, BN, BK, WM, WN>;
+  const __amdgpu_buffer_rsrc_t r = SL::rsrc(p, slot);
+#pragma unroll
+  for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+    for (int j = 0; j < TS::SM; ++j)
+#pragma unroll
+      for (int q = 0; q < SL::NACC / SL::VEC; ++q) {
+        const int vq = (i * TS::SM + j) * (SL::NACC / SL::VEC) + q;
+        typename SL::V v;
+        v.bits = __builtin_amdgcn_raw_buffer_load_b128(r, ((w * SL::NV + vq) * 64 + lane) * 16, 0, CPOL);
+#pragma unroll
+        for (int e = 0; e < SL::VEC; ++e) acc[i][j][q * SL::VEC + e] += v.e[e];
+      }
+}
+
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__device__ inline void sk_publish(const GettParams<typename PrecTraits<P>::TG>& p,
+                                  typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                             [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                  int slot, int w, int lane) {
+  using TS = TileShape<P, BM, BN, BK, WM, WN>;
+  using SL = SkSlot<P, BM, BN, BK, WM, WN>;
+  using TG = typename PrecTraits<P>::TG;
+  const __amdgpu_buffer_rsrc_t r = SL::rsrc(p, slot);
+#pragma unroll
+  for (int i = 0; i < TS::SN; ++i)
+#pragma unroll
+    for (int j = 0; j < TS::SM; ++j)
+#pragma unroll
+      for (int q = 0; q < SL::NACC / SL::VEC; ++q) {
+        typename SL::V v;
+#pragma unroll
+        for (int e = 0; e < SL::VEC; ++e) v.e[e] = (TG)acc[i][j][q * SL::VEC + e];
+        const int vq = (i * TS::SM + j) * (SL::NACC / SL::VEC) + q;
+        __builtin_amdgcn_raw_buffer_store_b128(v.bits, r, ((w * SL::NV + vq) * 64 + lane) * 16, 0, 16 /* sc1 */);
+      }
+  wait_vm_lgkm0<0>();   // this wave's partial stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(p.flags + slot, 1u + xcc_id(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Adds the partial sums of the tile's earlier k-steps to the accumulators,
+// segment by segment from the nearest down (a fixed order: deterministic),
+// each after its flag is seen; the flag is cleared for the next launch.
+// Every segment waited for has a lower workgroup index in the same XCD group
+// and publishes before it waits for anything, so the waits end; past
+// kSkSpinLimit rounds the error word records it and the partial is added as
+// found.
+template <int P, int BM, int BN, int BK, int WM, int WN>
+__device__ inline void sk_gather(const GettParams<typename PrecTraits<P>::TG>& p,
+                                 typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
+                                                            [TileShape<P, BM, BN, BK, WM, WN>::SM],
+                                 const SkPiece& sk, int w, int lane) {
+  using TS = TileShape<P, BM, BN, BK, WM, WN>;
+  using SL = SkSlot<P, BM, BN, BK, WM, WN>;
+  for (int jj = sk.seg - 1; jj >= 0; --jj) {
+    const long long e = (long long)(jj + 1) * sk.units / sk.nseg;
+    if (e <= sk.first) break;
+    if ((long long)jj * sk.units / sk.nseg >= e) continue;   // empty segment
+    const int src = sk.xcd + sk.nx * jj;
+    if (threadIdx.x == 0) {
+      int n = 0;
+      while (__hip_atomic_load(p.flags + src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        if (++n >= kSkSpinLimit) {
+          __hip_atomic_fetch_or(p.flags + gridDim.x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    // every thread reads the (now set) flag: the producer's XCD
+    const bool same_xcd =
+        __hip_atomic_load(p.flags + src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u + xcc_id();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(p.flags + src, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (same_xcd) sk_add_partial<P, BM, BN, BK, WM, WN, 1 /* sc0 */>(p, acc, src, w, lane);
+    else sk_add_partial<P, BM, BN, BK, WM, WN, 16 /* sc1 */>(p, acc, src, w, lane);
+  }
+}
+
 // The walk of one workgroup: tile (tm, tn), k-steps [k0, k0 + KT) (all of
-// them without a split; part ks of a split-k plan), and with PERSIST the
-// tiles of work items v + gridDim.x, ... < nblocks after it.  Every copy
-// into LDS has landed when it returns.
-template <int P, int BM, int BN, int BK, int WM, int WN, int S, bool PERSIST>
+// them without a split; part ks of a split-k plan; a stream-k piece with
+// SK), and with PERSIST the tiles of work items v + gridDim.x, ... < nblocks
+// after it.  Every copy into LDS has landed when it returns.
+template <int P, int BM, int BN, int BK, int WM, int WN, int S, bool PERSIST, bool SK = false>
 __device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits<P>::TG>& p, int v, int nblocks,
-                                            int tm, int tn, int ks, int k0, int KT, unsigned char* smem_raw) {
+                                            int tm, int tn, int ks, int k0, int KT, unsigned char* smem_raw,
+                                            const SkPiece& sk = SkPiece{}) {
+  static_assert(!(SK && PERSIST), "a stream-k piece is one tile's k range");
   using SH = PackedShape<P, BM, BN, BK, S>;
   using TS = TileShape<P, BM, BN, BK, WM, WN>;
   using MM = Mma<P>;
@@ -363,6 +513,13 @@ __device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits
   for (int i = 0; i < TS::SN; ++i)
 #pragma unroll
     for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
+  // A stream-k piece that finishes its tile starts from the earlier pieces'
+  // sums: their loads go out before the ring's first copies, which the
+  // prologue's wait covers (vmcnt completes in issue order), so their latency
+  // hides behind the copies'.
+  if constexpr (SK) {
+    if (sk.combine) sk_gather<P, BM, BN, BK, WM, WN>(p, acc, sk, w, lane);
+  }
 
   // Fragment double-buffering (the next step's fragments read during this
   // step's MFMAs) except for FP8, whose fragments are twice as large: there
@@ -463,6 +620,12 @@ __device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits
       step(f0, f1);
       if constexpr (DB) f0 = f1;  // the next tile's first fragments
     }
+    if constexpr (SK) {
+      if (sk.produce) {
+        sk_publish<P, BM, BN, BK, WM, WN>(p, acc, sk.slot, w, lane);
+        break;
+      }
+    }
     if (!PERSIST && ksplit > 1) {
       // raw sums of this k part into the partial-tile buffer, rows n of BM
       using TG = typename PrecTraits<P>::TG;
@@ -524,6 +687,88 @@ gett_packed_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   const int k0 = (int)((long)ks * p.kTiles / ksplit);
   const int KT = (int)((long)(ks + 1) * p.kTiles / ksplit) - k0;
   packed_walk<P, BM, BN, BK, WM, WN, S, PERSIST>(p, v, nblocks, tm, tn, ks, k0, KT, smem_raw);
+}
+
+// Whole rounds of tiles the stream-k walk leaves to stream-k besides the
+// last, partial one: the rest run whole, one per workgroup and round.
+constexpr int kSkTailRounds = 1;
+
+// Stream-k walk: the same k-step work spread evenly over the grid, so that
+// no round of tiles leaves CUs idle (the reference divides its loops over
+// threads by cost, tccg/gett.py:755-959; here the unit is one tile's
+// k-step).  With R = tiles / grid whole rounds, the first R - kSkTailRounds
+// rounds are walked tile by tile, workgroup g taking tiles g, g + grid, ...
+// of the XCD-chunked grouped order (neighbouring tiles on one XCD share
+// their blocks in its L2).  The remaining tiles go to the 8 XCDs (fewer
+// groups for a grid under 8) in contiguous whole-tile ranges; group x's
+// k-steps (tile-major) are cut into equal segments, one per workgroup x,
+// x + 8, ..., each walking its pieces from its last tile back to its first.
+// A piece that ends inside its tile publishes its raw sums (the first piece
+// of the segment walked); the piece that holds the tile's last k-step starts
+// from the earlier pieces' sums (it is the segment's last piece walked) and
+// writes C with alpha and beta once.  Every workgroup walks the same number
+// of whole tiles first, so the segments start together, and with at least
+// one round of tiles left to stream-k every segment holds a whole tile's
+// k-steps: the pieces a segment's last piece adds were published while its
+// earlier pieces ran.
+template <int P, int BM, int BN, int BK, int WM, int WN, int S>
+__global__ void __launch_bounds__(256)
+gett_packed_sk_kernel(GettParams<typename PrecTraits<P>::TG> p) {
+  extern __shared__ __align__(1024) unsigned char smem_raw[];
+  const int G = (int)gridDim.x;
+  const int g = blockIdx.x;
+  const int ntiles = p.tilesM * p.tilesN;
+  const int rounds = ntiles / G;
+  const int dp = rounds > kSkTailRounds ? (rounds - kSkTailRounds) * G : 0;   // tiles walked whole
+  const int ndp = dp / G;
+  // XCD groups: 8, or one per workgroup of a smaller grid (every group needs one)
+  const int NX = G < 8 ? G : 8;
+  const int x = g % NX, seg = g / NX;
+  const int nseg = (G - x + NX - 1) / NX;
+  const long long KT = p.kTiles;
+  const long long tail = ntiles - dp;
+  const long long T0 = dp + (long long)x * tail / NX, T1 = dp + (long long)(x + 1) * tail / NX;
+  const long long units = (T1 - T0) * KT;
+  const long long s = (long long)seg * units / nseg, e = (long long)(seg + 1) * units / nseg;
+  const int nsk = s < e ? (int)((e - 1) / KT - s / KT + 1) : 0;
+  SkPiece sk;
+  sk.slot = g;
+  sk.nx = NX;
+  sk.xcd = x;
+  sk.seg = seg;
+  sk.nseg = nseg;
+  sk.units = units;
+  for (int i = 0; i < ndp + nsk; ++i) {
+    int tm, tn, k0, nk;
+    if (i < ndp) {
+      tile_coords(xcd_chunked(g + i * G, dp), p.tilesM, p.tilesN, p.groupM, tm, tn);
+      k0 = 0;
+      nk = p.kTiles;
+      sk.produce = 0;
+      sk.combine = 0;
+    } else {
+      const long long t = (e - 1) / KT - (i - ndp);
+      const long long lo = s > t * KT ? s : t * KT;
+      const long long hi = e < (t + 1) * KT ? e : (t + 1) * KT;
+      sk.first = t * KT;
+      sk.produce = hi < (t + 1) * KT;
+      sk.combine = lo > t * KT && !sk.produce;
+      tile_coords((int)(T0 + t), p.tilesM, p.tilesN, p.groupM, tm, tn);
+      k0 = (int)(lo - t * KT);
+      nk = (int)(hi - lo);
+    }
+    // the previous piece's LDS reads (fragments, staged C) are done before
+    // this piece's copies overwrite the ring
+    if (i) __syncthreads();
+    packed_walk<P, BM, BN, BK, WM, WN, S, false, true>(p, 0, ntiles, tm, tn, 0, k0, nk, smem_raw, sk);
+  }
+}
+
+// Clears the stream-k flags and the error word (n words) before a launch
+// that follows a packing call; every launch leaves the flags clear itself.
+template <class W>
+__global__ void __launch_bounds__(256) sk_clear_kernel(W* flags, int n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) flags[i] = W(0);
 }
 
 // C = alpha * (sum of the ksplit partial tiles) + beta * C, the partial sums

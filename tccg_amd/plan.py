@@ -344,7 +344,8 @@ class GettPlan:
     packed: bool = False   # operands converted once into tile-major planes
     walk: int = 0          # 2: streaming walk over `grid` workgroups (single k-step);
                            # 3: vector micro-kernel (tile indexes VECTOR_TILES);
-                           # 4: packed, store-wave kernel over `grid` workgroups
+                           # 4: packed, store-wave kernel over `grid` workgroups;
+                           # 5: packed, stream-k walk over `grid` workgroups
     crun: Tuple[int, int] = (0, 0)  # packed: (E0, E1) blocks of C written as contiguous runs
     cvec: int = 0          # store-wave kernel: C written in 16-byte groups of four m positions
     ksplit: int = 1        # packed, one tile per workgroup: k range split into this many parts
@@ -375,6 +376,11 @@ class GettPlan:
             return 0
         per = _PLANES[self.prec] * _ELEM_BYTES[self.prec] * self.Kp
         ab = (-(-(per * self.Mp) // 256) * 256) + per * self.Np
+        if self.walk == 5 and self.grid > 0:
+            # one partial tile per workgroup, then a flag word per workgroup
+            # and the error word
+            parts = -(-(self.grid * self.BM * self.BN * (8 if self.prec == F64 else 4)) // 256) * 256
+            return -(-ab // 256) * 256 + parts + -(-((self.grid + 1) * 4) // 256) * 256 + PACKED_TAIL_BYTES
         if self.ksplit > 1:
             ab = -(-ab // 256) * 256 + self.ksplit * self.Mp * self.Np * (8 if self.prec == F64 else 4)
         return ab + PACKED_TAIL_BYTES
@@ -389,6 +395,8 @@ class GettPlan:
                                                " runs=%d" % self.crun[0] if self.crun[0] else "")
         elif self.walk == 2:
             walk = " stream grid=%d" % self.grid
+        elif self.walk == 5:
+            walk = " streamk grid=%d" % self.grid
         elif self.grid:
             walk = " persistent grid=%d" % self.grid
         if self.crun[0] and self.walk != 4:
@@ -475,14 +483,18 @@ def _store_run(offCm: np.ndarray, offCn: np.ndarray, BM: int) -> int:
 
 
 def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
-                      persistent: bool = False, store_waves: bool = False, ksplit: int = 1) -> GettPlan:
+                      persistent: bool = False, store_waves: bool = False, ksplit: int = 1,
+                      stream_k: int = 0) -> GettPlan:
     """Plan of the packed path: the reference's pack-then-macro-kernel GETT
     (tccg/gett.py:346-391, tccg/gett.py:1092-1098) with each operand packed
     once.  Plain linearisations (the pack kernel reads every element once, so
     no blocking is needed for the operand loads); m in C's order so C is
     written along its stride-one index; Kp padded to the pack block.  modeA /
     modeB hold the pack kernel's read order: KDIM1 when the operand's
-    stride-one index is contracted, else FREE1."""
+    stride-one index is contracted, else FREE1.  stream_k > 0: the stream-k
+    walk over stream_k workgroups per CU (every k-step of every tile spread
+    evenly over the grid; the reference balances its loops over threads by
+    cost, tccg/gett.py:755-959)."""
     BM, BN, BK = PACKED_TILES[prec][tile]
     m_l, n_l, k_l = _orders(pb)
     lin_m = Linearization.plain(m_l, _ext(pb, m_l))
@@ -522,6 +534,14 @@ def build_packed_plan(pb: GettProblem, prec: int, tile: int, groupM: int = 8,
         esz = 8 if prec == F64 else 4
         ring = packed_ring_bytes(prec, BM, BN, BK)
         plan.crun = c_runs(offCm, offCn, BM, BN, esz, ring // ((BM + 16 // esz) * esz))
+    if stream_k:
+        if persistent:
+            raise ValueError("the stream-k walk and the persistent walk exclude each other")
+        if not 1 <= stream_k <= 4:
+            raise ValueError("stream_k is the workgroups per CU, 1 to 4")
+        plan.walk = 5
+        plan.grid = MI355X["cus"] * stream_k
+        return plan
     if persistent:
         per_cu = max(1, min(2, MI355X["lds_bytes_per_cu"] // packed_ring_bytes(prec, BM, BN, BK)))
         plan.grid = min(plan.ntiles(), MI355X["cus"] * per_cu)
@@ -669,7 +689,8 @@ def build_plan(pb: GettProblem, prec: int, tile: int, m_blocked: Optional[bool] 
 # Plan variants as stored by the tuning database: bit 0 packed operands,
 # bit 1 persistent walk, bit 2 streaming walk, bit 3 vector micro-kernel.
 KIND_PACKED, KIND_PERSISTENT, KIND_STREAM, KIND_VECTOR, KIND_STORE_WAVES = 1, 2, 4, 8, 16
-KIND_KSPLIT_SHIFT = 8   # bits 8.. of a plan kind: the ksplit of a split-k plan
+KIND_KSPLIT_SHIFT = 8   # bits 8..11 of a plan kind: the ksplit of a split-k plan
+KIND_SK_SHIFT = 12      # bits 12..14: workgroups per CU of a stream-k plan
 
 
 def plan_kind(plan: GettPlan) -> int:
@@ -678,7 +699,9 @@ def plan_kind(plan: GettPlan) -> int:
     kind = KIND_PACKED if plan.packed else 0
     if plan.ksplit > 1:
         kind |= plan.ksplit << KIND_KSPLIT_SHIFT
-    if plan.walk == 4:
+    if plan.walk == 5:
+        kind |= (plan.grid // MI355X["cus"]) << KIND_SK_SHIFT
+    elif plan.walk == 4:
         kind |= KIND_STORE_WAVES
     elif plan.walk == 2:
         kind |= KIND_STREAM
@@ -711,7 +734,8 @@ def plan_from_kind(pb: GettProblem, prec: int, tile: int, m_blocked: bool, kind:
     if kind & KIND_PACKED:
         return build_packed_plan(pb, prec, tile, groupM=groupM, persistent=bool(kind & KIND_PERSISTENT),
                                  store_waves=bool(kind & KIND_STORE_WAVES),
-                                 ksplit=max(1, kind >> KIND_KSPLIT_SHIFT))
+                                 ksplit=max(1, (kind >> KIND_KSPLIT_SHIFT) & 0xf),
+                                 stream_k=(kind >> KIND_SK_SHIFT) & 0x7)
     return build_plan(pb, prec, tile, m_blocked=m_blocked, groupM=groupM,
                       persistent=bool(kind & KIND_PERSISTENT), stream=bool(kind & KIND_STREAM))
 
@@ -757,7 +781,7 @@ def _c_segment_bytes(plan: GettPlan) -> float:
                 adj += 1
             return min(1024.0, plan.crun[0] * adj * esz)
         return min(L, plan.BM) * esz if plan.cvec else esz
-    if plan.packed and plan.crun[0] and not plan.grid:
+    if plan.packed and plan.crun[0] and (not plan.grid or plan.walk == 5):
         return min(1024.0, plan.crun[0] * plan.crun[1] * esz)
     sub = 16 if plan.prec in (FP8, F64) else 32   # MFMA output columns along m
     return min(L, sub) * esz
@@ -853,7 +877,15 @@ def _estimate(plan: GettPlan, beta: float = 0.0) -> Tuple[float, float]:
     slots, t_mfma, t_fixed, t_pack, t_store = _tile_costs(plan, beta)
     ntiles = plan.ntiles() * plan.ksplit
     rounds = ntiles / slots if plan.grid else math.ceil(ntiles / slots)
-    if plan.walk == 4:
+    if plan.walk == 5:
+        # the tile work spread evenly over the machine; each workgroup walks
+        # about one piece more than its share of tiles (a pipeline start and
+        # an epilogue each) and publishes and adds one partial tile, mostly
+        # behind other workgroups' MFMAs
+        t_part = 2.0 * esz * plan.BM * plan.BN / (0.6 * hw["hbm_bytes_per_s"] / plan.grid)
+        t = (ntiles * t_mfma / slots + (ntiles / plan.grid + 1.0) * t_fixed * min(1.0, slots / plan.grid)
+             + 0.25 * t_part + t_pack)
+    elif plan.walk == 4:
         # store waves write C while the compute waves run the next tile: the
         # slower of the two per tile, plus staging the tile through LDS
         t = rounds * (max(t_mfma, t_store) + 0.3e-6) + t_pack
@@ -907,6 +939,13 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
             for ks in (2, 3, 4):
                 if ks <= kt // 8:
                     plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, ksplit=ks))
+        # stream-k when whole rounds of tiles would leave CUs idle in the last
+        # one: at least one round, few rounds, a partial last round, and k
+        # long enough that the pieces stay long
+        if MI355X["cus"] <= nt < 8 * MI355X["cus"] and nt % MI355X["cus"] and kt >= 8:
+            plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, stream_k=1))
+            if packed_ring_bytes(prec, BM, BN, BK) * 2 <= MI355X["lds_bytes_per_cu"]:
+                plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, stream_k=2))
     # ties at the HBM floor are broken by the time the tile work alone would take
     plans.sort(key=lambda p: _estimate(p))
     return plans

@@ -209,10 +209,12 @@ class DevicePlan:
         c.tile, c.modeA, c.modeB, c.groupM = plan.tile, plan.modeA, plan.modeB, plan.groupM
         c.grid = plan.grid
         c.walk = plan.walk
-        # the run epilogue is built for one tile per workgroup; the store-wave
-        # kernel (walk 4) reads crun_m as its run length
+        # the run epilogue is built for one tile (or one stream-k piece) at a
+        # time; the store-wave kernel (walk 4) reads crun_m as its run length
         if plan.walk == 4:
             c.crun_m, c.crun_n = plan.crun[0], 0
+        elif plan.walk == 5:
+            c.crun_m, c.crun_n = plan.crun
         else:
             c.crun_m, c.crun_n = plan.crun if not plan.grid else (0, 0)
         c.cvec = plan.cvec
