@@ -82,16 +82,34 @@ def seeded_inputs(c, dt, dist: str = "uniform", seed: int = 0):
     return A, B
 
 
+SETTLE_SECONDS = 0.05
+
+
+def settle(fn, warmup: int) -> int:
+    """The untimed warmup: `warmup` calls, then more until the GPU has run
+    calls for SETTLE_SECONDS, so that its clocks have left the idle state
+    the host-side tuning left them in (measured on MI355X: the first 20 ms
+    of calls after idle run up to 13 % slower).  Returns the calls made."""
+    import torch
+    n = 0
+    t0 = time.perf_counter()
+    while n < warmup or time.perf_counter() - t0 < SETTLE_SECONDS:
+        fn()
+        n += 1
+        if n >= warmup:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    return n
+
+
 def timed_steps(dp, A, B, C, steps: int, warmup: int):
     """(wall seconds of `steps` back-to-back calls bracketed by synchronize,
     average per-call GPU time in seconds from HIP events recorded on the
     launch stream before the first and after the last call: the GPU stays
-    busy, so host launch latency is not counted)."""
+    busy, so host launch latency is not counted, and the warmup calls made)."""
     import torch
     s = torch.cuda.current_stream()
-    for _ in range(warmup):
-        dp.run(A, B, C)
-    torch.cuda.synchronize()
+    nwarm = settle(lambda: dp.run(A, B, C), warmup)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(s)
@@ -100,7 +118,7 @@ def timed_steps(dp, A, B, C, steps: int, warmup: int):
     e1.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return wall, e0.elapsed_time(e1) / steps * 1e-3
+    return wall, e0.elapsed_time(e1) / steps * 1e-3, nwarm
 
 
 def graph_call_time(fn, C, steps: int, reps: int = 3):
@@ -205,7 +223,7 @@ def run_form(name, wl, prec, args, log):
         best = ms[0].plan
     tune_s = time.perf_counter() - t0
     dp = DevicePlan(best)
-    wall, call = timed_steps(dp, A, B, C, args.steps, args.warmup)
+    wall, call, nwarm = timed_steps(dp, A, B, C, args.steps, args.warmup)
     # GPU time per call and per GETT kernel without host gaps between launches
     g_call = graph_call_time(lambda: dp.run(A, B, C), C, args.steps)
     if g_call is not None:
@@ -223,7 +241,8 @@ def run_form(name, wl, prec, args, log):
     res = {
         "workload": str(c), "prec": prec, "M": c.problem.M, "N": c.problem.N, "K": c.problem.K,
         "plan": best.describe(), "tuned_candidates": len(ms), "tune_seconds": round(tune_s, 2),
-        "wall_ms_per_step": wall / args.steps * 1e3, "call_ms": call * 1e3, "kernel_ms": kern * 1e3,
+        "wall_ms_per_step": wall / args.steps * 1e3, "warmup_calls": nwarm, "call_ms": call * 1e3,
+        "kernel_ms": kern * 1e3,
         "cold_min_ms": cold * 1e3, "cold_median_ms": cold_med * 1e3,
         "gflops": flops / (wall / args.steps) / 1e9, "kernel_gflops": flops / kern / 1e9,
         "kernel_gbs": bytes_ / kern / 1e9, "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
@@ -618,6 +637,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        # the warmup calls made: at least `warmup`, more until the GPU had run
+        # them for SETTLE_SECONDS (clocks out of their idle state)
+        "warmup_calls": head["warmup_calls"],
         "ms_per_step": round(head["wall_ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",

@@ -271,17 +271,23 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
         kfA = 1 if plan.modeA in (2, 3) else 0
         kfB = 1 if plan.modeB in (2, 3) else 0
         amax = "t->amax" if plan.prec == FP8 else "nullptr"
+        flags = ("reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % _flags_offset(plan)
+                 if plan.walk == 5 else "nullptr")
         out.append("""  %(TL)s* wA = reinterpret_cast<%(TL)s*>(t->ws);
   %(TL)s* wB = reinterpret_cast<%(TL)s*>(reinterpret_cast<unsigned char*>(t->ws) + %(offB)d);
-  hipLaunchKernelGGL((tccg::pack_kernel<%(P)s, %(BM)d, %(BK)d>), dim3(%(gA)d, %(gK)d), dim3(256), 0, s, opA, t->Am, t->Ak,
-                     %(KT)d, %(kfA)d, %(amax)s, wA);
-  hipLaunchKernelGGL((tccg::pack_kernel<%(P)s, %(BN)d, %(BK)d>), dim3(%(gB)d, %(gK)d), dim3(256), 0, s, opB, t->Bn, t->Bk,
-                     %(KT)d, %(kfB)d, %(amaxB)s, wB);
+  // both operands in one launch, which also clears the stream-k flags
+  tccg::PackPair<%(T)s, %(TL)s> q;
+  q.X[0] = opA; q.X[1] = opB; q.offF[0] = t->Am; q.offF[1] = t->Bn; q.offK[0] = t->Ak; q.offK[1] = t->Bk;
+  q.amax[0] = %(amax)s; q.amax[1] = %(amaxB)s; q.out[0] = wA; q.out[1] = wB;
+  q.kfast[0] = %(kfA)d; q.kfast[1] = %(kfB)d; q.nF[0] = %(gA)d; q.nF[1] = %(gB)d; q.nK = %(gK)d; q.kTiles = %(KT)d;
+  q.clear = %(flags)s; q.nclear = %(nclear)d;
+  hipLaunchKernelGGL((tccg::pack_pair_kernel<%(P)s, %(BM)d, %(BN)d, %(BK)d>), dim3(%(blocks)d), dim3(256), 0, s, q);
   opA = reinterpret_cast<const %(T)s*>(wA);
   opB = reinterpret_cast<const %(T)s*>(wB);
 """ % dict(TL=TL, offB=offB, P=_PREC_ENUM[plan.prec], BM=plan.BM, BN=plan.BN, BK=plan.BK, gA=plan.Mp // 64,
            gB=plan.Np // 64, gK=plan.Kp // 32, KT=plan.Kp // plan.BK, kfA=kfA, kfB=kfB, amax=amax,
-           amaxB=(amax + " + 1") if plan.prec == FP8 else "nullptr", T=T))
+           amaxB=(amax + " + 1") if plan.prec == FP8 else "nullptr", T=T, flags=flags,
+           nclear=plan.grid + 1 if plan.walk == 5 else 0, blocks=(plan.Mp // 64 + plan.Np // 64) * (plan.Kp // 32)))
     out.append("""  tccg::GettParams<%s> p;
   p.A = opA; p.B = opB; p.C = C;
   p.offAm = t->Am; p.offAk = t->Ak; p.offBn = t->Bn; p.offBk = t->Bk; p.offCm = t->Cm; p.offCn = t->Cn;
@@ -291,7 +297,7 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
   p.amax = %s;
   p.crunM = %d; p.crunN = %d; p.cvec = %d;
   p.ksplit = %d; p.part = %s; p.flags = %s;
-%s  hipLaunchKernelGGL(kern, dim3(%d), dim3(%d), kLds, s, p);
+  hipLaunchKernelGGL(kern, dim3(%d), dim3(%d), kLds, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("kernel launch: ") + hipGetErrorString(e));
 %s  return 0;
@@ -302,9 +308,6 @@ __global__ void amax_reduce(const float* __restrict__ x, int64_t n, unsigned* o)
        plan.ksplit, _part_expr(plan, T),
        ("reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(t->ws) + %d)" % _flags_offset(plan)
         if plan.walk == 5 else "nullptr"),
-       ("  // the stream-k flags start clear\n"
-        "  hipLaunchKernelGGL(tccg::sk_clear_kernel<uint32_t>, dim3(%d), dim3(256), 0, s, p.flags, %d);\n"
-        % (-(-(plan.grid + 1) // 256), plan.grid + 1) if plan.walk == 5 else ""),
        plan.grid if plan.grid else plan.ntiles() * plan.ksplit, 512 if plan.walk == 4 else 256,
        _reduce_launch(plan, T)))
     out.append("void gett({T}* A, {T}* B, {T}* C, const {T} alpha, const {T} beta) {{\n"

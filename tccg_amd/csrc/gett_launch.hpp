@@ -514,40 +514,45 @@ int dispatch_packed_tile(int tile, const tccg::GettParams<typename tccg::PrecTra
   }
 }
 
-template <int P, int F, int BK>
-int launch_pack(const typename tccg::PrecTraits<P>::TG* X, const uint32_t* offF, const uint32_t* offK, int Fp,
-                int Kp, int kfast, const float* amax, typename tccg::PrecTraits<P>::TL* out, hipStream_t s) {
-  dim3 grid(Fp / tccg::kPackTF, Kp / tccg::kPackTK);
-  hipLaunchKernelGGL((tccg::pack_kernel<P, F, BK>), grid, dim3(256), 0, s, X, offF, offK, Kp / BK, kfast, amax, out);
+// Packed GETT: pack A and B (each once, into `workspace`; one launch, which
+// also clears `nclear` words at `clear`), then the GETT kernel over the
+// packed planes.  Both launches go to `stream`.
+template <int P, int BM, int BN, int BK>
+int run_packed_shape(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
+                     const tccg_gett_plan* plan, typename tccg::PrecTraits<P>::TL* wA,
+                     typename tccg::PrecTraits<P>::TL* wB, uint32_t* clear, int nclear, hipStream_t s) {
+  tccg::PackPair<typename tccg::PrecTraits<P>::TG, typename tccg::PrecTraits<P>::TL> q;
+  q.X[0] = A; q.X[1] = B;
+  q.offF[0] = plan->offAm; q.offF[1] = plan->offBn;
+  q.offK[0] = plan->offAk; q.offK[1] = plan->offBk;
+  q.amax[0] = plan->amax; q.amax[1] = plan->amax ? plan->amax + 1 : nullptr;
+  q.out[0] = wA; q.out[1] = wB;
+  q.kfast[0] = (plan->modeA == TCCG_KDIM4 || plan->modeA == TCCG_KDIM1) ? 1 : 0;
+  q.kfast[1] = (plan->modeB == TCCG_KDIM4 || plan->modeB == TCCG_KDIM1) ? 1 : 0;
+  q.nF[0] = plan->Mp / tccg::kPackTF;
+  q.nF[1] = plan->Np / tccg::kPackTF;
+  q.nK = plan->Kp / tccg::kPackTK;
+  q.kTiles = plan->Kp / BK;
+  q.clear = clear;
+  q.nclear = clear ? nclear : 0;
+  const long blocks = (long)(q.nF[0] + q.nF[1]) * q.nK;
+  if (blocks > 0x7fffffff) return fail(1, "too many pack blocks");
+  hipLaunchKernelGGL((tccg::pack_pair_kernel<P, BM, BN, BK>), dim3((unsigned)blocks), dim3(256), 0, s, q);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail((int)e, std::string("pack launch: ") + hipGetErrorString(e));
   return 0;
 }
 
-// Packed GETT: pack A, pack B (each once, into `workspace`), then the GETT
-// kernel over the packed planes.  All three launches go to `stream`.
-template <int P, int BM, int BN, int BK>
-int run_packed_shape(const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
-                     const tccg_gett_plan* plan, typename tccg::PrecTraits<P>::TL* wA,
-                     typename tccg::PrecTraits<P>::TL* wB, hipStream_t s) {
-  const int kfA = (plan->modeA == TCCG_KDIM4 || plan->modeA == TCCG_KDIM1) ? 1 : 0;
-  const int kfB = (plan->modeB == TCCG_KDIM4 || plan->modeB == TCCG_KDIM1) ? 1 : 0;
-  int rc = launch_pack<P, BM, BK>(A, plan->offAm, plan->offAk, plan->Mp, plan->Kp, kfA, plan->amax, wA, s);
-  if (rc) return rc;
-  return launch_pack<P, BN, BK>(B, plan->offBn, plan->offBk, plan->Np, plan->Kp, kfB,
-                                plan->amax ? plan->amax + 1 : nullptr, wB, s);
-}
-
 template <int P, int I = 0>
 int pack_operands(int tile, const typename tccg::PrecTraits<P>::TG* A, const typename tccg::PrecTraits<P>::TG* B,
                   const tccg_gett_plan* plan, typename tccg::PrecTraits<P>::TL* wA,
-                  typename tccg::PrecTraits<P>::TL* wB, hipStream_t s) {
+                  typename tccg::PrecTraits<P>::TL* wB, uint32_t* clear, int nclear, hipStream_t s) {
   if constexpr (I >= packed_count(P)) {
     return fail(1, "invalid packed tile id");
   } else {
     constexpr TileCfg c = packed_tile(P, I);
-    if (tile == I) return run_packed_shape<P, c.BM, c.BN, c.BK>(A, B, plan, wA, wB, s);
-    return pack_operands<P, I + 1>(tile, A, B, plan, wA, wB, s);
+    if (tile == I) return run_packed_shape<P, c.BM, c.BN, c.BK>(A, B, plan, wA, wB, clear, nclear, s);
+    return pack_operands<P, I + 1>(tile, A, B, plan, wA, wB, clear, nclear, s);
   }
 }
 
@@ -641,21 +646,14 @@ int run_gett_packed(const typename tccg::PrecTraits<P>::TG* A, const typename tc
       return fail(1, "the run epilogue (crun_m > 0) needs a 16-byte aligned C");
   }
   if (pack) {
-    int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, s);
+    // the stream-k flags start clear: the pack launch clears them
+    int rc = pack_operands<P>(plan->tile, A, B, plan, wA, wB, prm.flags, plan->grid + 1, s);
     if (rc) return rc;
   }
   if (plan->walk == 4) {
     if constexpr (std::is_same<TG, float>::value) return dispatch_packed_sw_tile<P>(plan->tile, prm, plan->grid, s);
   }
-  if (plan->walk == 5) {
-    if (pack) {
-      hipLaunchKernelGGL(tccg::sk_clear_kernel<uint32_t>, dim3((plan->grid + 1 + 255) / 256), dim3(256), 0, s, prm.flags,
-                         plan->grid + 1);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return fail((int)e, std::string("flag clear launch: ") + hipGetErrorString(e));
-    }
-    return dispatch_packed_sk_tile<P>(plan->tile, prm, plan->grid, s);
-  }
+  if (plan->walk == 5) return dispatch_packed_sk_tile<P>(plan->tile, prm, plan->grid, s);
   return dispatch_packed_tile<P>(plan->tile, prm, plan->grid, s);
 }
 

@@ -179,4 +179,42 @@ pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __
   pack_block<P, F, BK>(X, offF, offK, kTiles, kfast, amax, out, f0, k0, tile);
 }
 
+// Both operands of one contraction in one launch (one launch fewer per call,
+// and no gap between the two): logical blocks [0, nA * nK) pack A (64-row
+// blocks of FA-row tiles), the rest B, free position fastest, the grid
+// mapped XCD by XCD as in pack_kernel.  Block 0 also clears `nclear` words
+// at `clear` (the stream-k flags, which the GETT kernel launched after this
+// one reads).
+template <class TG, class TL>
+struct PackPair {
+  const TG* X[2];
+  const uint32_t* offF[2];
+  const uint32_t* offK[2];
+  const float* amax[2];
+  TL* out[2];
+  int kfast[2];
+  int nF[2];        // 64-row blocks along each operand's free set
+  int nK;           // 32-deep blocks along k
+  int kTiles;
+  uint32_t* clear;
+  int nclear;
+};
+
+template <int P, int FA, int FB, int BK>
+__global__ void __launch_bounds__(256)
+pack_pair_kernel(PackPair<typename PrecTraits<P>::TG, typename PrecTraits<P>::TL> q) {
+  __shared__ typename PrecTraits<P>::TG tile[kPackTF][kPackTK + 1];
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < q.nclear; i += 256) q.clear[i] = 0u;
+  const int nA = q.nF[0] * q.nK;
+  int lb = xcd_chunked(blockIdx.x, gridDim.x);
+  const int op = lb < nA ? 0 : 1;
+  if (op) lb -= nA;
+  const int f0 = (lb % q.nF[op]) * kPackTF, k0 = (lb / q.nF[op]) * kPackTK;
+  if (op == 0)
+    pack_block<P, FA, BK>(q.X[0], q.offF[0], q.offK[0], q.kTiles, q.kfast[0], q.amax[0], q.out[0], f0, k0, tile);
+  else
+    pack_block<P, FB, BK>(q.X[1], q.offF[1], q.offK[1], q.kTiles, q.kfast[1], q.amax[1], q.out[1], f0, k0, tile);
+}
+
 }  // namespace tccg

@@ -307,6 +307,7 @@ __device__ inline void store_tile_staged(const GettParams<typename PrecTraits<P>
 struct SkPiece {
   int produce;      // the tile's later k-steps belong to the next segment: publish the raw sums
   int combine;      // the tile's earlier k-steps belong to earlier segments: add their sums, then store
+  int gather_first; // combine: add them before this piece's k-steps (else after)
   int slot;         // this workgroup's partial slot and flag (its workgroup index)
   int nx;           // XCD groups (8, fewer for a grid under 8)
   int xcd, seg, nseg;
@@ -513,12 +514,14 @@ __device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits
   for (int i = 0; i < TS::SN; ++i)
 #pragma unroll
     for (int j = 0; j < TS::SM; ++j) acc[i][j] = typename MM::Acc{};
-  // A stream-k piece that finishes its tile starts from the earlier pieces'
-  // sums: their loads go out before the ring's first copies, which the
-  // prologue's wait covers (vmcnt completes in issue order), so their latency
-  // hides behind the copies'.
+  // A stream-k piece that finishes its tile adds the earlier pieces' sums.
+  // When its segment holds at least a tile's k-steps those were published
+  // while the segment's earlier pieces ran: it starts from them, their loads
+  // going out before the ring's first copies, which the prologue's wait
+  // covers (vmcnt completes in issue order), so their latency hides behind
+  // the copies'.  A shorter segment adds them after its own k-steps.
   if constexpr (SK) {
-    if (sk.combine) sk_gather<P, BM, BN, BK, WM, WN>(p, acc, sk, w, lane);
+    if (sk.combine && sk.gather_first) sk_gather<P, BM, BN, BK, WM, WN>(p, acc, sk, w, lane);
   }
 
   // Fragment double-buffering (the next step's fragments read during this
@@ -621,6 +624,7 @@ __device__ __forceinline__ void packed_walk(const GettParams<typename PrecTraits
       if constexpr (DB) f0 = f1;  // the next tile's first fragments
     }
     if constexpr (SK) {
+      if (sk.combine && !sk.gather_first) sk_gather<P, BM, BN, BK, WM, WN>(p, acc, sk, w, lane);
       if (sk.produce) {
         sk_publish<P, BM, BN, BK, WM, WN>(p, acc, sk.slot, w, lane);
         break;
@@ -704,13 +708,13 @@ constexpr int kSkTailRounds = 1;
 // k-steps (tile-major) are cut into equal segments, one per workgroup x,
 // x + 8, ..., each walking its pieces from its last tile back to its first.
 // A piece that ends inside its tile publishes its raw sums (the first piece
-// of the segment walked); the piece that holds the tile's last k-step starts
-// from the earlier pieces' sums (it is the segment's last piece walked) and
+// of the segment walked); the piece that holds the tile's last k-step adds
+// the earlier pieces' sums (it is the segment's last piece walked) and
 // writes C with alpha and beta once.  Every workgroup walks the same number
-// of whole tiles first, so the segments start together, and with at least
-// one round of tiles left to stream-k every segment holds a whole tile's
-// k-steps: the pieces a segment's last piece adds were published while its
-// earlier pieces ran.
+// of whole tiles first, so the segments start together; when every segment
+// holds at least a whole tile's k-steps, the pieces a segment's last piece
+// adds were published while its earlier pieces ran, and it adds them before
+// its own k-steps.
 template <int P, int BM, int BN, int BK, int WM, int WN, int S>
 __global__ void __launch_bounds__(256)
 gett_packed_sk_kernel(GettParams<typename PrecTraits<P>::TG> p) {
@@ -738,6 +742,7 @@ gett_packed_sk_kernel(GettParams<typename PrecTraits<P>::TG> p) {
   sk.seg = seg;
   sk.nseg = nseg;
   sk.units = units;
+  sk.gather_first = units / nseg >= KT;
   for (int i = 0; i < ndp + nsk; ++i) {
     int tm, tn, k0, nk;
     if (i < ndp) {
@@ -762,13 +767,6 @@ gett_packed_sk_kernel(GettParams<typename PrecTraits<P>::TG> p) {
     if (i) __syncthreads();
     packed_walk<P, BM, BN, BK, WM, WN, S, false, true>(p, 0, ntiles, tm, tn, 0, k0, nk, smem_raw, sk);
   }
-}
-
-// Clears the stream-k flags and the error word (n words) before a launch
-// that follows a packing call; every launch leaves the flags clear itself.
-template <class W>
-__global__ void __launch_bounds__(256) sk_clear_kernel(W* flags, int n) {
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) flags[i] = W(0);
 }
 
 // C = alpha * (sum of the ksplit partial tiles) + beta * C, the partial sums

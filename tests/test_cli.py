@@ -189,5 +189,6 @@ def test_emit_packed_source():
     plan = build_packed_plan(c.problem, BF16X3, 0)
     src = codegen.emit_source(plan, WORKLOADS["example"], "gett")
     assert "gett_packed_kernel<tccg::BF16X3, 128, 128, 16" in src
-    assert src.count("tccg::pack_kernel<tccg::BF16X3") == 2
+    # both operands packed in one launch
+    assert src.count("tccg::pack_pair_kernel<tccg::BF16X3, 128, 128, 16>") == 1
     assert "hipMalloc(&t.ws, %d)" % plan.workspace_bytes() in src
