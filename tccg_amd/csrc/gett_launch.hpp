@@ -51,7 +51,7 @@ constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192
 // and BN is a multiple of the 64-row pack block.
 constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
                                        {192, 192, 16, 2, 2},  {128, 64, 16, 2, 2},  {64, 128, 16, 2, 2},
-                                       {192, 64, 16, 2, 2}};
+                                       {192, 64, 16, 2, 2},   {192, 192, 32, 2, 2}};
 constexpr TileCfg kF64PackedTiles[] = {{128, 128, 16, 2, 2}, {64, 64, 16, 2, 2}};
 constexpr TileCfg kBfPackedTiles[] = {{128, 128, 16, 2, 2}, {128, 128, 32, 2, 2}, {256, 128, 16, 2, 2},
                                       {128, 256, 16, 2, 2}, {64, 64, 32, 2, 2},   {192, 192, 16, 2, 2},

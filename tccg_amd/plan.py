@@ -53,7 +53,7 @@ PACKED_TAIL_BYTES = 160 * 1024
 
 PACKED_TILES: Dict[int, List[Tuple[int, int, int]]] = {
     F32: [(128, 128, 32), (256, 128, 16), (128, 128, 16), (192, 192, 16), (128, 64, 16), (64, 128, 16),
-          (192, 64, 16)],
+          (192, 64, 16), (192, 192, 32)],
     F64: [(128, 128, 16), (64, 64, 16)],
     BF16X3: [(128, 128, 16), (128, 128, 32), (256, 128, 16), (128, 256, 16), (64, 64, 32), (192, 192, 16),
              (192, 64, 16)],
