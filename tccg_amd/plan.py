@@ -940,9 +940,11 @@ def candidate_plans(pb: GettProblem, prec: int, groupM: int = 8) -> List[GettPla
                 if ks <= kt // 8:
                     plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, ksplit=ks))
         # stream-k when whole rounds of tiles would leave CUs idle in the last
-        # one: at least one round, few rounds, a partial last round, and k
-        # long enough that the pieces stay long
-        if MI355X["cus"] <= nt < 8 * MI355X["cus"] and nt % MI355X["cus"] and kt >= 8:
+        # one: few rounds, a partial last round, and k long enough that the
+        # pieces stay long (under one round, k long enough to give every
+        # workgroup a few k-steps of a tile)
+        if ((MI355X["cus"] <= nt < 8 * MI355X["cus"] and nt % MI355X["cus"] and kt >= 8)
+                or (nt < MI355X["cus"] and nt * kt >= 8 * MI355X["cus"])):
             plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, stream_k=1))
             if packed_ring_bytes(prec, BM, BN, BK) * 2 <= MI355X["lds_bytes_per_cu"]:
                 plans.append(build_packed_plan(pb, prec, tile, groupM=groupM, stream_k=2))

@@ -184,9 +184,16 @@ def test_stream_k_plans_and_kinds():
     # offered where whole rounds of tiles leave the last one partly idle
     walks = [(p.walk, p.grid) for p in candidate_plans(c.problem, F32)]
     assert (5, 256) in walks and (5, 512) in walks
-    # not for a contraction that fills under one round of the CUs
+    # not for a contraction whose tiles and k-steps give the grid little work
     small = Contraction.from_tccg(get("cpu48"))
     assert all(p.walk != 5 for p in candidate_plans(small.problem, F32))
+    # offered under one round of tiles when k is long (next to split-k): the
+    # first adjoint of abcd-aklb-lckd has 162 tiles of 192 x 192, k = 5184
+    from tccg_amd.autograd import adjoints
+    dA, _ = adjoints(Contraction.from_tccg(get("abcd-aklb-lckd")))
+    plans = candidate_plans(dA.problem, F32)
+    assert any(p.walk == 5 and p.ntiles() < MI355X["cus"] for p in plans)
+    assert any(p.ksplit > 1 for p in plans)
 
 
 def test_stream_k_workspace_bytes_agree():
