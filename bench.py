@@ -102,6 +102,23 @@ def settle(fn, warmup: int) -> int:
     return n
 
 
+def refine(ms, A, B, C, steps: int):
+    """The fastest of the autotuner's best few plans by graph-replayed calls
+    (pack kernels + GETT, `steps` per replay, the minimum of three, each
+    replay checked against an eager call): the eager timings of the
+    autotuner tell plans within a percent or two apart less reliably."""
+    from tccg_amd.runtime import DevicePlan
+    best, best_t = ms[0].plan, float("inf")
+    for m in ms:
+        dp = DevicePlan(m.plan)
+        settle(lambda: dp.run(A, B, C), 2)
+        t = graph_call_time(lambda: dp.run(A, B, C), C, steps)
+        if t is not None and t < best_t:
+            best, best_t = m.plan, t
+        del dp
+    return best
+
+
 def timed_steps(dp, A, B, C, steps: int, warmup: int):
     """(wall seconds of `steps` back-to-back calls bracketed by synchronize,
     average per-call GPU time in seconds from HIP events recorded on the
@@ -220,7 +237,7 @@ def run_form(name, wl, prec, args, log):
     else:
         ms = autotune(c, p, max_implementations=args.max_implementations, reps=3, inputs=(A, B, C),
                       verbose=args.verbose)
-        best = ms[0].plan
+        best = refine(ms[:3], A, B, C, args.steps)
     tune_s = time.perf_counter() - t0
     dp = DevicePlan(best)
     wall, call, nwarm = timed_steps(dp, A, B, C, args.steps, args.warmup)
