@@ -316,8 +316,10 @@ struct SkPiece {
 };
 
 // Spin limit of a combining workgroup's wait for one partial (each round
-// sleeps about 128 cycles): about 0.2 s, far beyond any producer's lead.
-constexpr int kSkSpinLimit = 1 << 21;
+// sleeps about 128 cycles, so about 0.5 s at 2.4 GHz): a producer publishes
+// its piece of at most one tile's k-steps before anything else, so it is
+// reached only by a piece that long taking as long, or by a fault.
+constexpr int kSkSpinLimit = 1 << 23;
 
 // Partial slot layout: 16-byte vectors of VEC consecutive accumulator
 // elements in register order; vector q of wave w and lane l at byte
