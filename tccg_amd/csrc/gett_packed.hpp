@@ -780,28 +780,33 @@ __global__ void __launch_bounds__(256) split_reduce_kernel(GettParams<TG> p) {
   constexpr int QM = BM / 4;
   const long total = (long)ntiles * BN * QM;
   const size_t tile_elems = (size_t)BN * BM, part_elems = (size_t)ntiles * tile_elems;
-  for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
-    const int tile = (int)(g / (BN * QM));
-    const int rem = (int)(g - (long)tile * BN * QM);
-    const int n = rem / QM, m4 = (rem - n * QM) * 4;
-    const int tm = tile / p.tilesN, tn = tile - tm * p.tilesN;
-    const uint32_t cn = p.offCn[tn * BN + n];
-    if (cn == kInvalid) continue;
-    const TG* src = p.part + (size_t)tile * tile_elems + (size_t)n * BM + m4;
-    TG s[4] = {TG(0), TG(0), TG(0), TG(0)};
-    for (int ks = 0; ks < p.ksplit; ++ks) {
+  // one loop per beta case, so that with beta == 0 nothing reads C
+  auto reduce = [&](auto beta0) {
+    for (long g = (long)blockIdx.x * 256 + threadIdx.x; g < total; g += (long)gridDim.x * 256) {
+      const int tile = (int)(g / (BN * QM));
+      const int rem = (int)(g - (long)tile * BN * QM);
+      const int n = rem / QM, m4 = (rem - n * QM) * 4;
+      const int tm = tile / p.tilesN, tn = tile - tm * p.tilesN;
+      const uint32_t cn = p.offCn[tn * BN + n];
+      if (cn == kInvalid) continue;
+      const TG* src = p.part + (size_t)tile * tile_elems + (size_t)n * BM + m4;
+      TG s[4] = {TG(0), TG(0), TG(0), TG(0)};
+      for (int ks = 0; ks < p.ksplit; ++ks) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) s[j] += src[(size_t)ks * part_elems + j];
-    }
+        for (int j = 0; j < 4; ++j) s[j] += src[(size_t)ks * part_elems + j];
+      }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t cm = p.offCm[tm * BM + m4 + j];
-      if (cm == kInvalid) continue;
-      TG* dst = p.C + (size_t)(cm + cn);
-      if (p.beta == TG(0)) __builtin_nontemporal_store(p.alpha * s[j], dst);
-      else *dst = p.alpha * s[j] + p.beta * (*dst);
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t cm = p.offCm[tm * BM + m4 + j];
+        if (cm == kInvalid) continue;
+        TG* dst = p.C + (size_t)(cm + cn);
+        if constexpr (decltype(beta0)::value) *dst = p.alpha * s[j];
+        else *dst = p.alpha * s[j] + p.beta * (*dst);
+      }
     }
-  }
+  };
+  if (p.beta == TG(0)) reduce(std::true_type{});
+  else reduce(std::false_type{});
 }
 
 }  // namespace tccg
