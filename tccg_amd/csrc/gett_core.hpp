@@ -500,7 +500,20 @@ __device__ inline void wait_vm_lgkm0() {
 // completed (the offset loads and whatever was in flight before them), so
 // that no later wait of this wave has to count them; the C stores are not
 // waited for.
-template <int P, int BM, int BN, int BK, int WM, int WN, bool DRAIN = false>
+//
+// WB (the persistent walks): with beta == 0 the 4-byte stores are ordinary
+// write-back stores.  The planner picks persistent walks for many tiles of
+// few k-steps, where writing C bounds the kernel; where C's contiguous runs
+// are short (24 elements on the precision contraction) a store instruction
+// covers parts of several 128-byte lines, and L2 merges a tile's parts of
+// a line before writing it back, where non-temporal stores send each part
+// to HBM on its own (measured on that contraction: BF16 364 -> 151 us,
+// split-BF16 275 -> 253 us).  Otherwise (one tile per workgroup, stream-k
+// pieces) they are non-temporal: C then leaves no dirty lines in L2 and the
+// Infinity Cache whose write-back would slow the next call's pack launch
+// (measured on the compute-bound contraction: pack launch plus gap 37 us
+// with non-temporal stores, 55 us with write-back ones).
+template <int P, int BM, int BN, int BK, int WM, int WN, bool DRAIN = false, bool WB = DRAIN>
 __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::TG>& p,
                                        typename Mma<P>::Acc (&acc)[TileShape<P, BM, BN, BK, WM, WN>::SN]
                                                                   [TileShape<P, BM, BN, BK, WM, WN>::SM],
@@ -531,13 +544,6 @@ __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::
   // consecutive addresses along C's stride-one index) are stored back to
   // back.  Common case first: beta == 0 and every position of the wave's
   // sub-tile holds an element, so the stores need no per-element checks.
-  // The 4-byte stores are ordinary (write-back) stores, not non-temporal
-  // ones: where C's contiguous runs are short (24 elements on the
-  // precision contraction) a store instruction covers parts of several
-  // 128-byte lines, and L2 merges the tile's parts of a line before writing
-  // it back, where non-temporal stores send each part to HBM on its own
-  // (measured on that contraction: BF16 364 -> 151 us, split-BF16 275 ->
-  // 253 us; no change where the runs are long).
   bool valid = true;
 #pragma unroll
   for (int j = 0; j < S::SM; ++j) valid = valid && cm[j] != kInvalid;
@@ -554,7 +560,9 @@ __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::
         for (int j = 0; j < S::SM; ++j) {
           TG val = (TG)acc[i][j][r];
           if (P == FP8) val = (TG)((float)val * descale);
-          p.C[(size_t)(cm[j] + cn[i][r])] = alpha * val;
+          TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
+          if constexpr (WB) *dst = alpha * val;
+          else __builtin_nontemporal_store(alpha * val, dst);
         }
     return;
   }
@@ -570,8 +578,9 @@ __device__ inline void store_tile_from(const GettParams<typename PrecTraits<P>::
             TG val = (TG)acc[i][j][r];
             if (P == FP8) val = (TG)((float)val * descale);
             TG* dst = p.C + (size_t)(cm[j] + cn[i][r]);
-            if constexpr (decltype(beta0)::value) *dst = alpha * val;
-            else *dst = alpha * val + beta * (*dst);
+            if constexpr (!decltype(beta0)::value) *dst = alpha * val + beta * (*dst);
+            else if constexpr (WB) *dst = alpha * val;
+            else __builtin_nontemporal_store(alpha * val, dst);
           }
   };
   if (beta == TG(0)) update(std::true_type{});
@@ -823,6 +832,7 @@ __device__ inline void gett_walk(const GettParams<typename PrecTraits<P>::TG>& p
 
     const TG alpha = p.alpha, beta = p.beta;
     // one loop nest per beta case, so that with beta == 0 nothing reads C
+    // (a persistent walk: write-back stores, as store_tile_from's WB)
     auto update = [&](auto beta0) {
 #pragma unroll
       for (int j = 0; j < S::SM; ++j)

@@ -800,7 +800,7 @@ __global__ void __launch_bounds__(256) split_reduce_kernel(GettParams<TG> p) {
         const uint32_t cm = p.offCm[tm * BM + m4 + j];
         if (cm == kInvalid) continue;
         TG* dst = p.C + (size_t)(cm + cn);
-        if constexpr (decltype(beta0)::value) *dst = p.alpha * s[j];
+        if constexpr (decltype(beta0)::value) __builtin_nontemporal_store(p.alpha * s[j], dst);
         else *dst = p.alpha * s[j] + p.beta * (*dst);
       }
     }
