@@ -568,8 +568,8 @@ def cpu_baseline(args, log):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the GETT backend runs one contraction per GPU and this build "
-              "has no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the GETT backend runs one contraction per GPU and has "
+              "no multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():

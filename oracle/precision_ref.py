@@ -1,8 +1,8 @@
 """CPU emulations of the reduced-precision GETT forms (test infrastructure).
 
 This file is part of the oracle: only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg use it, as the checker.  The forms have no
-counterpart in the reference (north_star's precision forms); each emulation
+bench.py's cpu_baseline leg use it, as the checker.  The forms (split-BF16,
+FP8 e4m3 with per-tensor scales) have no counterpart in the reference; each emulation
 restates the arithmetic the GPU kernels are specified to perform, with torch
 CPU casts for the roundings, and sums the products in FP64:
 
