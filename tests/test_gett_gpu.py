@@ -90,6 +90,19 @@ def test_alpha_beta(prec, alpha, beta):
     check(c, prec, seed=3, alpha=alpha, beta=beta)
 
 
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
+@pytest.mark.parametrize("alpha,beta", [(2.4, 1.1), (0.5, 0.0)])
+def test_gather_persistent_alpha_beta(prec, alpha, beta):
+    """The gather kernel's persistent walk (a grid below the tile count;
+    write-back C stores): beta != 0 and beta == 0, ragged edge tiles."""
+    c = Contraction.from_labels(["a", "b", "c"], ["a", "k", "c"], ["k", "b"], {"a": 37, "b": 70, "c": 9, "k": 45})
+    plan = gather_plan(c, prec)
+    plan.grid = min(3, plan.ntiles() - 1)
+    assert plan.grid > 0
+    print("gather persistent", prec, alpha, beta, plan.describe(), flush=True)
+    check(c, prec, seed=5, alpha=alpha, beta=beta, plan=plan)
+
+
 def test_beta_from_tccg():
     txt = """C[i1,j,i2] = 2.4 * A[i1,k,i2] * B[j,k] + 1.1 * C[i1,j,i2]
 i1 = 48

@@ -95,6 +95,21 @@ def test_packed_persistent_every_tile(prec, steps):
         check(c, prec, seed=tile + 10 * steps, plan=plan, tol=_tol(prec))
 
 
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3, FP8])
+@pytest.mark.parametrize("alpha,beta", [(2.4, 1.1), (1.0, -2.0), (0.5, 0.0)])
+def test_packed_persistent_alpha_beta(prec, alpha, beta):
+    """The persistent walk's epilogue (write-back C stores): C read and
+    updated with beta != 0, and with beta == 0 both the whole tiles (no
+    per-element checks) and the ragged edge tiles."""
+    BM, BN, _ = PACKED_TILES[prec][0]
+    c = Contraction.from_labels(["a", "b"], ["a", "k"], ["k", "b"], {"a": 2 * BM + 17, "b": 3 * BN + 5, "k": 61})
+    plan = build_packed_plan(c.problem, prec, 0)
+    plan.grid = 5
+    assert plan.ntiles() > plan.grid
+    print("persistent alpha/beta", prec, alpha, beta, plan.describe(), flush=True)
+    check(c, prec, seed=21, alpha=alpha, beta=beta, plan=plan, tol=_tol(prec))
+
+
 # (precision, packed tile id, unpacked tile id) with the same (BM, BN, BK)
 SAME_SHAPE = [(F32, 0, 0), (F32, 2, 5), (F64, 0, 0), (BF16X3, 1, 0), (BF16X3, 0, 3), (BF16, 0, 0), (FP8, 0, 0)]
 
