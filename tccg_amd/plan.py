@@ -161,7 +161,9 @@ class GettProblem:
         self.m = get_m_indices(A, B, C)
         self.n = get_n_indices(A, B, C)
         self.k = get_contracted_indices(A, B, C)
-        self.swapped = bool(self.n) and A.has_index(self.n[0])
+        # the internal A operand holds C's stride-one index (and with it the
+        # m set), also when the n set is empty (tccg/tccg_util.py:233-238)
+        self.swapped = not A.has_index(C.indices[0])
         self.opA, self.opB = (B, A) if self.swapped else (A, B)
         self.M, self.N, self.K = product(self.m), product(self.n), product(self.k)
 

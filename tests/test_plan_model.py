@@ -70,6 +70,45 @@ def test_every_plan_addresses_the_contraction(seed):
     assert any(p for p, _, _ in kinds) and any(not p for p, _, _ in kinds), kinds
 
 
+@pytest.mark.parametrize("seed", range(24))
+def test_random_index_sets_every_plan(seed):
+    """The random cases of the GPU sweep (any of the m, n, k sets may be
+    empty, index orders shuffled, ragged extents): every candidate plan's
+    tables address the contraction, and the internal A operand holds C's
+    stride-one index (with an empty n set it may be either input)."""
+    from .random_cases import random_contraction
+    c = random_contraction(seed)
+    rng = np.random.default_rng(seed)
+    sa, sb, sc = c.shapes()
+    A, B, C = rng.uniform(-1, 1, sa), rng.uniform(-1, 1, sb), rng.uniform(-1, 1, sc)
+    labels = lambda T: [i.label for i in T.indices]
+    ref = contract_ref(labels(c.C0), labels(c.A0), labels(c.B0), A, B, C, 1.25, -0.5)
+    pb = c.problem
+    assert pb.opA.has_index(pb.C.indices[0]) and not pb.opB.has_index(pb.C.indices[0])
+    for prec in (F32, F64):
+        for plan in candidate_plans(pb, prec):
+            out = emulate(plan, A, B, C, 1.25, -0.5)
+            assert rel_frobenius(out, ref) < 1e-12, (str(c), plan.describe())
+
+
+def test_vector_operand_in_either_input():
+    """C[j] = A[k] B[k, j] and C[j] = A[k, j] B[k]: the matrix operand is the
+    internal A operand in both, whichever input it is."""
+    for c_l, a_l, b_l in ((["j"], ["k"], ["k", "j"]), (["j"], ["k", "j"], ["k"]),
+                          (["j", "i"], ["k", "h"], ["h", "i", "k", "j"])):
+        c = Contraction.from_labels(c_l, a_l, b_l, {"j": 37, "k": 21, "i": 5, "h": 3})
+        pb = c.problem
+        assert pb.opA.has_index(pb.C.indices[0]) and pb.N == 1, str(pb)
+        assert pb.swapped == ("j" not in a_l)
+        rng = np.random.default_rng(len(a_l))
+        sa, sb, sc = c.shapes()
+        A, B, C = rng.uniform(-1, 1, sa), rng.uniform(-1, 1, sb), rng.uniform(-1, 1, sc)
+        labels = lambda T: [i.label for i in T.indices]
+        ref = contract_ref(labels(c.C0), labels(c.A0), labels(c.B0), A, B, C, 2.0, 0.5)
+        for plan in candidate_plans(pb, F32):
+            assert rel_frobenius(emulate(plan, A, B, C, 2.0, 0.5), ref) < 1e-12, plan.describe()
+
+
 def emulate_store_waves(plan, C_exact, C_init):
     """The store-wave kernel's write of a finished tile: in run order (crun
     R > 0) the group of four m positions m4 of a run starting at r0 goes to
