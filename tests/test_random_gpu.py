@@ -1,34 +1,43 @@
 """Seeded random contractions through every candidate plan, against the FP64
-einsum oracle.
+einsum oracle (FP8: against its e4m3 emulation).
 
 Each case draws the index sets (free indices of A only, of B only,
 contracted ones; any of them may be empty as long as A, B and C keep an
 index), the order of every tensor's indices, ragged extents and alpha/beta,
 then runs every plan the planner offers for it (gather, blocked, persistent
 and streaming walks, vector kernel, packed tiles, persistent, store-wave,
-split-k and stream-k plans) in FP32, FP64 and split-BF16 (tests/random_cases.py;
+split-k and stream-k plans) in every precision (tests/random_cases.py;
 tests/test_plan_model.py checks the same cases' tables on the CPU).
 """
 import numpy as np
 import pytest
 
-from tccg_amd.plan import BF16X3, F32, F64, candidate_plans
+from tccg_amd.plan import BF16, BF16X3, F32, F64, FP8, candidate_plans
 
 from .gpu_helpers import TOL, inputs, run_gpu
 from .random_cases import random_contraction
 
 pytestmark = pytest.mark.gpu
 
-@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
-@pytest.mark.parametrize("seed", range(24))
+TOL_BF16 = 1e-2   # BF16 against the FP64 einsum: one BF16 product per term
+TOL_FP8 = 1e-4    # FP8 against its emulation (the MFMA's group sums, about 1e-5)
+
+
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3, BF16, FP8])
+@pytest.mark.parametrize("seed", range(48))
 def test_random_contraction_every_plan(seed, prec):
     c = random_contraction(seed)
     alpha, beta = [(1.0, 0.0), (2.4, 1.1), (-0.5, 0.0)][seed % 3]
     dtype = np.float64 if prec == F64 else np.float32
     A, B, C = inputs(c, seed, "normal", dtype, with_c=beta != 0)
     from oracle.einsum_ref import contract_ref, rel_frobenius
+    from oracle.precision_ref import fp8_contract
     labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
-    ref = contract_ref(*labels, A, B, C, alpha, beta)
+    if prec == FP8:
+        # the e4m3 emulation (per-tensor scales, exact products, FP64 sum)
+        ref = fp8_contract(*labels, A, B, alpha) + (beta * C if beta != 0 else 0.0)
+    else:
+        ref = contract_ref(*labels, A, B, C, alpha, beta)
     plans = candidate_plans(c.problem, prec)
     assert plans
     worst = 0.0
@@ -36,6 +45,7 @@ def test_random_contraction_every_plan(seed, prec):
         print("random", seed, prec, c, i, plan.describe(), flush=True)
         out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
         err = rel_frobenius(out, ref)
-        assert np.isfinite(err) and err <= TOL[prec], "plan %d (%s): rel. error %.3e" % (i, plan.describe(), err)
+        limit = TOL_BF16 if prec == BF16 else TOL_FP8 if prec == FP8 else TOL[prec]
+        assert np.isfinite(err) and err <= limit, "plan %d (%s): rel. error %.3e" % (i, plan.describe(), err)
         worst = max(worst, err)
     print("random", seed, prec, len(plans), "plans, worst rel. error %.2e" % worst, flush=True)
