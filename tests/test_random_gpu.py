@@ -49,3 +49,67 @@ def test_random_contraction_every_plan(seed, prec):
         assert np.isfinite(err) and err <= limit, "plan %d (%s): rel. error %.3e" % (i, plan.describe(), err)
         worst = max(worst, err)
     print("random", seed, prec, len(plans), "plans, worst rel. error %.2e" % worst, flush=True)
+
+
+def _tensors(c, seed, dtype, with_c):
+    import torch
+    A, B, C = inputs(c, seed, "normal", dtype, with_c=with_c)
+    dev = torch.device("cuda", 0)
+    return A, B, C, torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+@pytest.mark.parametrize("seed", range(48))
+def test_random_contraction_ttgt(seed, prec):
+    """TTGT (transposition kernels around the GETT) on the random cases: the
+    model's best candidates plus one that transposes C and one that exchanges
+    A and B, with beta == 0 and beta != 0 (candidate bookkeeping is modelled
+    on the CPU for every candidate by tests/test_ttgt.py's restatement)."""
+    import torch
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    from tccg_amd.ttgt import DeviceTtgt, best_candidates, ttgt_candidates
+    c = random_contraction(seed)
+    dtype = np.float64 if prec == F64 else np.float32
+    A, B, C, tA, tB = _tensors(c, seed, dtype, True)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    cands = ttgt_candidates(c.problem, 8 if prec == F64 else 4)
+    chosen = best_candidates(cands, 3)
+    for pred in (lambda x: x.transC, lambda x: x.swap == 1):
+        extra = next((x for x in cands if pred(x)), None)
+        if extra is not None and extra not in chosen:
+            chosen.append(extra)
+    for cand in chosen:
+        d = DeviceTtgt(c, cand, prec)
+        for alpha, beta in ((1.0, 0.0), (1.5, -0.5)):
+            print("random ttgt", seed, prec, c, cand.name, alpha, beta, flush=True)
+            tC = torch.from_numpy(C).to(tA.device)
+            d.run(tA, tB, tC, alpha, beta)
+            torch.cuda.synchronize()
+            ref = contract_ref(*labels, A, B, C, alpha, beta)
+            assert rel_frobenius(tC.cpu().numpy(), ref) <= TOL[prec], (cand.name, alpha, beta)
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_contraction_log(seed):
+    """LoG (every GEMM instance one problem of a grouped launch) on the random
+    cases that have LoG candidates, every candidate, FP32 and FP64."""
+    import torch
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    from tccg_amd.loop_gemm import DeviceLoG, log_candidates
+    c = random_contraction(seed)
+    cands = log_candidates(c.problem)
+    if not cands:
+        pytest.skip("no LoG candidate for %s" % c)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    for prec in (F32, F64):
+        dtype = np.float64 if prec == F64 else np.float32
+        A, B, C, tA, tB = _tensors(c, seed, dtype, True)
+        for cand in cands:
+            for alpha, beta in ((1.0, 0.0), (1.5, -0.5)):
+                print("random log", seed, prec, c, cand.name, alpha, beta, flush=True)
+                tC = torch.from_numpy(C).to(tA.device)
+                d = DeviceLoG(c, cand, tA, tB, tC, alpha, beta, prec=prec)
+                d.launch()
+                torch.cuda.synchronize()
+                ref = contract_ref(*labels, A, B, C, alpha, beta)
+                assert rel_frobenius(tC.cpu().numpy(), ref) <= TOL[prec], (cand.name, prec, alpha, beta)

@@ -66,3 +66,47 @@ def test_candidates_compute_the_contraction(wl):
     cands = ttgt_candidates(c.problem)
     for cand in best_candidates(cands, 16) + cands[:: max(1, len(cands) // 12)]:
         assert rel_frobenius(_cpu_ttgt(c, cand, A, B, C, 1.7, 0.3), ref) < 1e-13, cand.name
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_cases_transpositions_and_gemm(seed):
+    """The random cases of the GPU sweep (tests/random_cases.py): for the
+    candidates the GPU test runs (the model's best three, one that transposes
+    C, one that exchanges A and B), every transposition plan through the
+    transposition kernels' addressing model (every access in bounds, every
+    output written once) against the transposition oracle, and the GEMM's
+    default plan through the GETT addressing model, in FP32 and FP64 layouts."""
+    from tccg_amd.plan import F32, F64
+    from tccg_amd.transpose import plan_transpose
+    from .random_cases import random_contraction
+    from .test_plan_model import emulate as gemm_emulate
+    from .test_transpose import emulate as tr_emulate
+    c = random_contraction(seed)
+    pb = c.problem
+    sizes = {i.label: i.size for T in (pb.opA, pb.opB, pb.C) for i in T.indices}
+    rng = np.random.default_rng(seed)
+    for prec, esz in ((F32, 4), (F64, 8)):
+        cands = ttgt_candidates(pb, 8 if prec == F64 else 4)
+        chosen = best_candidates(cands, 3)
+        for pred in (lambda x: x.transC, lambda x: x.swap == 1):
+            extra = next((x for x in cands if pred(x)), None)
+            if extra is not None and extra not in chosen:
+                chosen.append(extra)
+        for cand in chosen:
+            X, Y = (pb.opB, pb.opA) if cand.swap else (pb.opA, pb.opB)
+            for need, src, dst in ((cand.transA, X.labels(), cand.idxA), (cand.transB, Y.labels(), cand.idxB),
+                                   (cand.transC, cand.idxC, pb.C.labels())):
+                if not need:
+                    continue
+                plan = plan_transpose(src, dst, sizes, elem_bytes=esz)
+                shape = [sizes[l] for l in reversed(list(src))]
+                T = rng.standard_normal(shape)
+                OUT = rng.standard_normal([sizes[l] for l in reversed(list(dst))])
+                got = tr_emulate(plan, T, OUT, 1.0, 0.0)
+                np.testing.assert_array_equal(got.reshape(-1), transpose_ref(src, dst, T).reshape(-1))
+            gemm = Contraction.from_labels(cand.idxC, cand.idxA, cand.idxB, sizes)
+            sa, sb, sc = gemm.shapes()
+            A, B, C = rng.standard_normal(sa), rng.standard_normal(sb), rng.standard_normal(sc)
+            ref = contract_ref(cand.idxC, cand.idxA, cand.idxB, A, B, C, 1.5, -0.5)
+            out = gemm_emulate(gemm.plan(prec), A, B, C, 1.5, -0.5)
+            assert rel_frobenius(out, ref) < 1e-12, (cand.name, prec)
