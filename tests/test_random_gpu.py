@@ -113,3 +113,34 @@ def test_random_contraction_log(seed):
                 torch.cuda.synchronize()
                 ref = contract_ref(*labels, A, B, C, alpha, beta)
                 assert rel_frobenius(tC.cpu().numpy(), ref) <= TOL[prec], (cand.name, prec, alpha, beta)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("seed", range(48))
+def test_random_contraction_autograd(seed, prec):
+    """The differentiable GETT on the random cases: the forward and both
+    adjoint GETTs (whose index sets may be empty where the forward's are not)
+    against torch.einsum's FP64 autograd on the CPU."""
+    import torch
+    from oracle.einsum_ref import rel_frobenius
+    from tccg_amd.autograd import contract
+    from .test_autograd_gpu import einsum_eq
+    c = random_contraction(seed)
+    g = torch.Generator().manual_seed(seed)
+    A64 = torch.randn(c.A0.torch_shape(), dtype=torch.float64, generator=g)
+    B64 = torch.randn(c.B0.torch_shape(), dtype=torch.float64, generator=g)
+    dC64 = torch.randn(c.C0.torch_shape(), dtype=torch.float64, generator=g)
+    Ar, Br = A64.clone().requires_grad_(), B64.clone().requires_grad_()
+    Cr = c.alpha * torch.einsum(einsum_eq(c), Ar, Br)
+    Cr.backward(dC64)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    A = A64.to(dt).cuda().requires_grad_()
+    B = B64.to(dt).cuda().requires_grad_()
+    print("random autograd", seed, prec, c, flush=True)
+    C = contract(A, B, c, prec)
+    C.backward(dC64.to(dt).cuda())
+    torch.cuda.synchronize()
+    tol = 1e-12 if prec == "f64" else 1e-5
+    for name, got, want in (("C", C.detach(), Cr.detach()), ("dA", A.grad, Ar.grad), ("dB", B.grad, Br.grad)):
+        err = rel_frobenius(got.cpu().double().numpy(), want.numpy())
+        assert err <= tol, (name, err)
