@@ -29,7 +29,19 @@ def test_record_layouts():
 
 @pytest.mark.parametrize("prec,tile", [(F32, 2), (F32, 3), (F64, 0)])
 def test_batch_records(prec, tile):
-    cs = random_batch(40, seed=3)
+    _check_records(random_batch(40, seed=3), prec, tile)
+
+
+@pytest.mark.parametrize("batch", range(6))
+def test_random_case_records(batch):
+    """Batches of eight of the GPU sweep's random contractions
+    (tests/random_cases.py: any of m, n, k empty, shuffled index orders,
+    ragged extents) on the grouped launch's records."""
+    from .random_cases import random_contraction
+    _check_records([random_contraction(8 * batch + i) for i in range(8)], F32, 2)
+
+
+def _check_records(cs, prec, tile):
     plans = [build_packed_plan(c.problem, prec, tile) for c in cs]
     As = [FakeTensor(0x10000000 * (i + 1)) for i in range(len(cs))]
     Bs = [FakeTensor(0x10000000 * (i + 1) + 0x4000000) for i in range(len(cs))]

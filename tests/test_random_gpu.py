@@ -144,3 +144,38 @@ def test_random_contraction_autograd(seed, prec):
     for name, got, want in (("C", C.detach(), Cr.detach()), ("dA", A.grad, Ar.grad), ("dB", B.grad, Br.grad)):
         err = rel_frobenius(got.cpu().double().numpy(), want.numpy())
         assert err <= tol, (name, err)
+
+
+@pytest.mark.parametrize("mode", ["persistent", "grouped"])
+@pytest.mark.parametrize("batch", range(6))
+def test_random_batch(batch, mode):
+    """Batches of eight random cases (different patterns and extents in one
+    launch) with per-problem alpha and beta: the persistent gather launch and
+    the grouped packed launch (records of the same batches checked on the CPU
+    by tests/test_group.py)."""
+    import torch
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    from tccg_amd.batch import GettBatch
+    from tccg_amd.plan import PACKED_TILES
+    cs = [random_contraction(8 * batch + i) for i in range(8)]
+    rng = np.random.default_rng(batch)
+    alphas = [float(x) for x in rng.uniform(-2, 2, len(cs))]
+    betas = [0.0 if i % 2 else float(x) for i, x in enumerate(rng.uniform(-1, 1, len(cs)))]
+    dev = torch.device("cuda", 0)
+    host, As, Bs, Cs = [], [], [], []
+    for c in cs:
+        A, B, C = (rng.standard_normal(s).astype(np.float32) for s in c.shapes())
+        host.append((A, B, C))
+        As.append(torch.from_numpy(A).to(dev))
+        Bs.append(torch.from_numpy(B).to(dev))
+        Cs.append(torch.from_numpy(C.copy()).to(dev))
+    kw = dict(group_tile=PACKED_TILES[F32].index((128, 128, 16))) if mode == "grouped" else {}
+    print("random batch", batch, mode, [str(c) for c in cs], flush=True)
+    b = GettBatch(cs, alphas=alphas, betas=betas, **kw).bind(As, Bs, Cs)
+    b.launch()
+    torch.cuda.synchronize()
+    for i, c in enumerate(cs):
+        labels = [[x.label for x in T.indices] for T in (c.C0, c.A0, c.B0)]
+        A, B, C = host[i]
+        ref = contract_ref(*labels, A, B, C, alphas[i], betas[i])
+        assert rel_frobenius(Cs[i].cpu().numpy(), ref) <= TOL[F32], (i, str(c))
