@@ -393,6 +393,11 @@ def packed_suffix(packed) -> str:
 SPLIT_K_TEXTS = {
     "aklb-abcd-lckd": "C[a,k,l,b] = A[a,b,c,d] * B[l,c,k,d]\na = 24\nb = 24\nc = 24\nd = 24\nk = 16\nl = 16\n",
 }
+# A contraction whose A has no free index (C's indices all come from B, the
+# n set is empty): the generated entry point exchanges the operands.
+VECTOR_A_TEXTS = {
+    "ji-kh-hikj": "C[j,i] = 1.5 * A[k,h] * B[h,i,k,j]\nj = 37\ni = 5\nk = 21\nh = 3\n",
+}
 
 
 def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) -> list:
@@ -412,6 +417,8 @@ def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) 
         # and split-k and stream-k plans on the first adjoint of abcd-aklb-lckd
         # (small M*N, long k)
         texts.update(SPLIT_K_TEXTS)
+        texts.update(VECTOR_A_TEXTS)
+        todo += [(wl, F32, None) for wl in VECTOR_A_TEXTS]
         todo += [("abcd-aebf-dfce", BF16X3, dict(tile=0, persistent=False)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=0, persistent=True)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=4, store_waves=True)),

@@ -42,6 +42,28 @@ def test_generated_workload_library(wl):
     assert rel_frobenius(tC2.cpu().numpy(), 2.5 * ref) < 1e-5
 
 
+def test_generated_library_vector_operand_a():
+    """C[j,i] = 1.5 * A[k,h] * B[h,i,k,j]: A holds no index of C, so the
+    generated gett() passes B as the internal A operand (empty n set)."""
+    import torch
+    wl = "ji-kh-hikj"
+    lib_path = codegen.GENERATED_DIR / ("lib%s.so" % codegen.workload_library_name(wl, F32))
+    assert lib_path.exists(), "generated library missing: run __graft_entry__.build()"
+    g = GeneratedGett(lib_path)
+    c = Contraction.from_tccg(codegen.VECTOR_A_TEXTS[wl])
+    assert c.problem.swapped and c.problem.N == 1
+    assert (g.info["M"], g.info["N"], g.info["K"]) == (c.problem.M, c.problem.N, c.problem.K)
+    A, B, C = inputs(c, seed=4, dist="normal", with_c=True)
+    dev = torch.device("cuda", 0)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    for alpha, beta in ((1.5, 0.0), (-0.5, 2.0)):
+        tA, tB, tC = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+        g(tA, tB, tC, alpha, beta)
+        torch.cuda.synchronize()
+        ref = contract_ref(*labels, A, B, C, alpha, beta)
+        assert rel_frobenius(tC.cpu().numpy(), ref) < 1e-5, (alpha, beta)
+
+
 def test_cli_testing_on_gpu(tmp_path):
     f = tmp_path / "transC.tccg"
     f.write_text("C[i1,j,i2] = 2.4 * A[i1,k,i2] * B[j,k] + 1.1 * C[i1,j,i2]\ni1 = 96\ni2 = 48\nj = 64\nk = 8\n")
