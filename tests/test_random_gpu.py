@@ -182,3 +182,28 @@ def test_random_batch(batch, mode):
         A, B, C = host[i]
         ref = contract_ref(*labels, A, B, C, alphas[i], betas[i])
         assert rel_frobenius(Cs[i].cpu().numpy(), ref) <= TOL[F32], (i, str(c))
+
+
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
+@pytest.mark.parametrize("seed", range(48))
+def test_random_contraction_stream_k(seed, prec):
+    """Stream-k plans (walk 5) of packed tile 0 on the random cases over 3, 37
+    and 256 workgroups: segments inside one tile, empty segments, grids under
+    the 8 XCD groups (the partition and partial-sum protocol of the same
+    (grid, tiles, k-steps) triples is modelled on the CPU by
+    tests/test_stream_k.py::test_random_case_partitions)."""
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    from tccg_amd.plan import build_packed_plan
+    c = random_contraction(seed)
+    alpha, beta = [(1.0, 0.0), (2.4, 1.1), (-0.5, 0.0)][seed % 3]
+    dtype = np.float64 if prec == F64 else np.float32
+    A, B, C = inputs(c, seed, "normal", dtype, with_c=beta != 0)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    ref = contract_ref(*labels, A, B, C, alpha, beta)
+    for grid in (3, 37, 256):
+        plan = build_packed_plan(c.problem, prec, 0, stream_k=1)
+        plan.grid = grid
+        print("random stream-k", seed, prec, grid, c, plan.describe(), flush=True)
+        out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
+        err = rel_frobenius(out, ref)
+        assert np.isfinite(err) and err <= TOL[prec], (grid, plan.describe(), err)

@@ -215,3 +215,17 @@ def test_stream_k_workspace_bytes_agree():
             n = lib.tccg_packed_workspace_bytes(prec, ctypes.byref(cp))
             esz = 8 if prec == F64 else 4
             assert n == p.workspace_bytes() >= p.grid * p.BM * p.BN * esz + 4 * (p.grid + 1)
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_case_partitions(seed):
+    """The (grid, tiles, k-steps) triples of the GPU sweep's stream-k plans
+    (tests/random_cases.py, packed tile 0, grids 3, 37 and 256) through the
+    partition and protocol checks above."""
+    from tccg_amd.plan import BF16X3
+    from .random_cases import random_contraction
+    c = random_contraction(seed)
+    for prec in (F32, F64, BF16X3):
+        p = build_packed_plan(c.problem, prec, 0, stream_k=1)
+        for grid in (3, 37, 256):
+            test_units_covered_once_and_protocol(grid, p.ntiles(), p.Kp // p.BK)
