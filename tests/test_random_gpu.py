@@ -207,3 +207,30 @@ def test_random_contraction_stream_k(seed, prec):
         out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
         err = rel_frobenius(out, ref)
         assert np.isfinite(err) and err <= TOL[prec], (grid, plan.describe(), err)
+
+
+@pytest.mark.parametrize("prec", [F32, F64, BF16X3])
+@pytest.mark.parametrize("seed", range(48))
+def test_random_contraction_split_k(seed, prec):
+    """Split-k plans (k parts into partial tiles, then the ordered reduction
+    writing alpha * sum + beta * C) of the 128 x 128 x 16 packed tile with 2
+    and 3 parts on the random cases, wherever the plan has that many k-steps
+    (the partition is modelled on the CPU by tests/test_plan_model.py)."""
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    from tccg_amd.plan import PACKED_TILES, build_packed_plan
+    c = random_contraction(seed)
+    alpha, beta = [(1.0, 0.0), (2.4, 1.1), (-0.5, 0.0)][seed % 3]
+    dtype = np.float64 if prec == F64 else np.float32
+    A, B, C = inputs(c, seed, "normal", dtype, with_c=beta != 0)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    ref = contract_ref(*labels, A, B, C, alpha, beta)
+    tile = PACKED_TILES[prec].index((128, 128, 16))
+    kt = build_packed_plan(c.problem, prec, tile).Kp // 16
+    for ks in (2, 3):
+        if ks > kt:
+            continue
+        plan = build_packed_plan(c.problem, prec, tile, ksplit=ks)
+        print("random split-k", seed, prec, ks, c, plan.describe(), flush=True)
+        out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
+        err = rel_frobenius(out, ref)
+        assert np.isfinite(err) and err <= TOL[prec], (ks, plan.describe(), err)
