@@ -61,7 +61,7 @@ enum Prec : int { F32 = 0, F64 = 1, BF16X3 = 2, FP8 = 3, BF16 = 4 };
 //   KDIM4 : stride-one index is the first index of the k set, extent % 4 == 0
 //           -> vector loads of 4 consecutive k, lanes along k.
 //   KDIM1 : lanes along k, scalar loads.
-// (Operands packed once by pack_kernel are read by gett_packed_kernel,
+// (Operands packed once by pack_pair_kernel are read by gett_packed_kernel,
 // gett_packed.hpp, not by this loader.)
 enum LoadMode : int { FREE4 = 0, FREE1 = 1, KDIM4 = 2, KDIM1 = 3 };
 

@@ -47,7 +47,7 @@ constexpr TileCfg kBf1Tiles[] = {{128, 128, 32, 2, 2}, {64, 64, 32, 2, 2}, {192,
 constexpr TileCfg kF8Tiles[] = {{128, 128, 128, 2, 2}, {64, 64, 128, 2, 2}, {192, 128, 128, 2, 2}};
 
 // Tile configurations of the packed path (operands converted once by
-// pack_kernel, then copied tile by tile); ids index these arrays.  Every BM
+// pack_pair_kernel, then copied tile by tile); ids index these arrays.  Every BM
 // and BN is a multiple of the 64-row pack block.
 constexpr TileCfg kF32PackedTiles[] = {{128, 128, 32, 2, 2}, {256, 128, 16, 2, 2}, {128, 128, 16, 2, 2},
                                        {192, 192, 16, 2, 2},  {128, 64, 16, 2, 2},  {64, 128, 16, 2, 2},

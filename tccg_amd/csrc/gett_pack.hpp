@@ -4,7 +4,7 @@
 // before its macro-kernel runs (packA/packB through HPTT,
 // tccg/gett.py:346-391 and tccg/gett.py:700-753; packed layout
 // tccg/gett.py:1092-1098).  This is the gfx950 form of that step, done once
-// per operand for the whole contraction: pack_kernel gathers the operand along
+// per operand for the whole contraction: pack_block gathers the operand along
 // its stride-one index, converts it to the arithmetic type of the MFMA
 // (FP32 or FP64 unchanged, three BF16 parts, one BF16 part, or per-tensor
 // scaled OCP e4m3) and writes tile-major planes that gett_packed_kernel
@@ -165,24 +165,12 @@ __device__ __forceinline__ void pack_block(const typename PrecTraits<P>::TG* __r
   PackWriter<P>::write8(out + blk, F, F * BK, gf % F, gk % BK, v, scale);
 }
 
-// Grid: (Fp / 64, Kp / 32), one block per workgroup.
-template <int P, int F, int BK>
-__global__ void __launch_bounds__(256)
-pack_kernel(const typename PrecTraits<P>::TG* __restrict__ X, const uint32_t* __restrict__ offF,
-            const uint32_t* __restrict__ offK, int kTiles, int kfast, const float* __restrict__ amax,
-            typename PrecTraits<P>::TL* __restrict__ out) {
-  __shared__ typename PrecTraits<P>::TG tile[kPackTF][kPackTK + 1];
-  // consecutive blocks along the free set run on one XCD: neighbours that
-  // share the partial cache lines of a stride-one run meet in the same L2
-  const int lb = xcd_chunked(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-  const int f0 = (lb % gridDim.x) * kPackTF, k0 = (lb / gridDim.x) * kPackTK;
-  pack_block<P, F, BK>(X, offF, offK, kTiles, kfast, amax, out, f0, k0, tile);
-}
-
 // Both operands of one contraction in one launch (one launch fewer per call,
 // and no gap between the two): logical blocks [0, nA * nK) pack A (64-row
 // blocks of FA-row tiles), the rest B, free position fastest, the grid
-// mapped XCD by XCD as in pack_kernel.  Block 0 also clears `nclear` words
+// mapped XCD by XCD (consecutive blocks along the free set run on one XCD:
+// neighbours that share the partial cache lines of a stride-one run meet in
+// the same L2).  Block 0 also clears `nclear` words
 // at `clear` (the stream-k flags, which the GETT kernel launched after this
 // one reads).
 template <class TG, class TL>

@@ -1,6 +1,6 @@
 // GETT main loop over packed operands on gfx950.
 //
-// pack_kernel (gett_pack.hpp) has written each operand once as tile-major
+// pack_pair_kernel (gett_pack.hpp) has written each operand once as tile-major
 // blocks: k-step kb of tile row block fb is BLOCK = PLANES * F * BK
 // consecutive elements at (fb * kTiles + kb) * BLOCK.  Inside a block each
 // plane is split into 32-byte sub-rows, [plane][k / KS][row][KS] with KS the
