@@ -398,6 +398,19 @@ SPLIT_K_TEXTS = {
 VECTOR_A_TEXTS = {
     "ji-kh-hikj": "C[j,i] = 1.5 * A[k,h] * B[h,i,k,j]\nj = 37\ni = 5\nk = 21\nh = 3\n",
 }
+# Index sets at their edges, each compiled with the planner's best plan: an
+# empty n set (a vector kernel), an extent-1 index among shuffled orders, A
+# without a free index (packed store waves, operands exchanged), fused
+# indices with exchanged operands, an empty k set (outer product).  (A
+# tensor with one index needs a label of two characters: the .tccg tensor
+# pattern, tccg/tccg.py:574, matches at least two.)
+EDGE_TEXTS = {
+    "heg-ehakg-ak": "C[h,e,g] = A[e,h,ak,g] * B[ak]\nh = 30\ne = 7\ng = 36\nak = 8\n",
+    "hge-ihed-dig": "C[h,g,e] = A[i,h,e,d] * B[d,i,g]\nh = 20\ng = 4\ne = 21\ni = 6\nd = 1\n",
+    "cg-jk-gjkc": "C[c,g] = A[jk] * B[g,jk,c]\nc = 42\ng = 11\njk = 26\n",
+    "jch-dchf-djf": "C[j,c,h] = A[d,c,h,f] * B[d,j,f]\nj = 32\nc = 25\nh = 24\nd = 7\nf = 25\n",
+    "icfe-fe-ci": "C[i,c,f,e] = A[f,e] * B[c,i]\ni = 42\nc = 5\nf = 43\ne = 23\n",
+}
 
 
 def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) -> list:
@@ -418,7 +431,8 @@ def build_workload_libraries(force: bool = False, workloads=None, precs=(F32,)) 
         # (small M*N, long k)
         texts.update(SPLIT_K_TEXTS)
         texts.update(VECTOR_A_TEXTS)
-        todo += [(wl, F32, None) for wl in VECTOR_A_TEXTS]
+        texts.update(EDGE_TEXTS)
+        todo += [(wl, F32, None) for wl in list(VECTOR_A_TEXTS) + list(EDGE_TEXTS)]
         todo += [("abcd-aebf-dfce", BF16X3, dict(tile=0, persistent=False)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=0, persistent=True)),
                  ("abcdef-dgfa-gceb", FP8, dict(tile=4, store_waves=True)),

@@ -64,6 +64,30 @@ def test_generated_library_vector_operand_a():
         assert rel_frobenius(tC.cpu().numpy(), ref) < 1e-5, (alpha, beta)
 
 
+@pytest.mark.parametrize("wl", sorted(codegen.EDGE_TEXTS))
+def test_generated_library_edge_index_sets(wl):
+    """Generated libraries of contractions at the edges of the index sets
+    (empty n or k set, extent-1 index, A without a free index, fused indices
+    with exchanged operands), each with the planner's best plan, through the
+    generated gett() with beta == 0 and beta != 0."""
+    import torch
+    lib_path = codegen.GENERATED_DIR / ("lib%s.so" % codegen.workload_library_name(wl, F32))
+    assert lib_path.exists(), "generated library missing: run __graft_entry__.build()"
+    g = GeneratedGett(lib_path)
+    c = Contraction.from_tccg(codegen.EDGE_TEXTS[wl])
+    assert (g.info["M"], g.info["N"], g.info["K"]) == (c.problem.M, c.problem.N, c.problem.K)
+    A, B, C = inputs(c, seed=8, dist="normal", with_c=True)
+    dev = torch.device("cuda", 0)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    for alpha, beta in ((1.0, 0.0), (0.75, -1.5)):
+        print("generated edge", wl, g.info, alpha, beta, flush=True)
+        tA, tB, tC = (torch.from_numpy(x).to(dev) for x in (A, B, C))
+        g(tA, tB, tC, alpha, beta)
+        torch.cuda.synchronize()
+        ref = contract_ref(*labels, A, B, C, alpha, beta)
+        assert rel_frobenius(tC.cpu().numpy(), ref) < 1e-5, (alpha, beta)
+
+
 def test_cli_testing_on_gpu(tmp_path):
     f = tmp_path / "transC.tccg"
     f.write_text("C[i1,j,i2] = 2.4 * A[i1,k,i2] * B[j,k] + 1.1 * C[i1,j,i2]\ni1 = 96\ni2 = 48\nj = 64\nk = 8\n")
