@@ -234,3 +234,22 @@ def test_random_contraction_split_k(seed, prec):
         out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
         err = rel_frobenius(out, ref)
         assert np.isfinite(err) and err <= TOL[prec], (ks, plan.describe(), err)
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+@pytest.mark.parametrize("seed", range(6))
+def test_random_alpha_zero_and_accumulate(seed, prec):
+    """alpha = 0 (C becomes beta * C, the products ignored) and beta = 1
+    (accumulation into C) through every candidate plan of a random case."""
+    from oracle.einsum_ref import contract_ref, rel_frobenius
+    c = random_contraction(seed)
+    dtype = np.float64 if prec == F64 else np.float32
+    A, B, C = inputs(c, seed, "normal", dtype, with_c=True)
+    labels = [[i.label for i in T.indices] for T in (c.C0, c.A0, c.B0)]
+    for alpha, beta in ((0.0, 1.5), (1.0, 1.0)):
+        ref = contract_ref(*labels, A, B, C, alpha, beta)
+        for i, plan in enumerate(candidate_plans(c.problem, prec)):
+            print("random alpha/beta", seed, prec, alpha, beta, c, i, plan.describe(), flush=True)
+            out = run_gpu(c, A, B, C, prec, alpha, beta, plan=plan)
+            err = rel_frobenius(out, ref)
+            assert np.isfinite(err) and err <= TOL[prec], (alpha, beta, plan.describe(), err)
